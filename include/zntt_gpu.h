@@ -1,0 +1,213 @@
+/*
+ * zntt_gpu.h -- C ABI of the MI355X (gfx950) NTT library, libzntt_gpu.so.
+ *
+ * The library is a drop-in backend for zig-algebra's `libs/ntt` (zig-ntt).
+ * Its seven core entry points are exactly the functions a Zig `GpuNtt` shim
+ * would bind through `extern fn`, one per zig-ntt function, with the same
+ * argument meaning, the same element layout and the same error sets:
+ *
+ *   zig-ntt (libs/ntt/src/root.zig)            C ABI
+ *   ----------------------------------------   -------------------------------
+ *   bitReverse          :38-58                 zntt_bit_reverse
+ *   ntt                 :65-90                 zntt_ntt
+ *   intt                :95-107                zntt_intt
+ *   precomputeTwiddles  :112-134               zntt_precompute_twiddles
+ *   freeTwiddles        :136-141               zntt_free_twiddles
+ *   nttWithTwiddles     :148-175               zntt_ntt_with_twiddles
+ *   inttWithTwiddles    :180-212               zntt_intt_with_twiddles
+ *
+ * Element layout (byte-identical to the Zig types, so the shim passes
+ * `data.ptr` straight through):
+ *   ZNTT_FIELD_BN254_FR   32 bytes: four little-endian u64 limbs of the
+ *                         Montgomery form x * 2^256 mod r (libs/field/src/
+ *                         field.zig:884, montgomery.zig:23-79), r from
+ *                         libs/curve/src/bn254.zig:67.
+ *   ZNTT_FIELD_GOLDILOCKS  8 bytes: the canonical u64 value
+ *                         (libs/field/src/field.zig:142), p = 2^64 - 2^32 + 1.
+ *
+ * Semantics:
+ *   - Host-buffer calls are synchronous and in place, like the Zig functions:
+ *     they return with `data` updated.  They are serialised per context.
+ *   - `root` is not validated (libs/ntt/README.md:151-152): outputs equal the
+ *     reference's radix-2 DIT DAG for every root, primitive or not.
+ *   - Error order follows the reference: LogTooLarge -> LengthMismatch ->
+ *     InvalidTwiddles (table count) -> InvalidTwiddles (stage lengths).
+ *     Deliberate deviation: every stage length is checked before `data` is
+ *     touched (the reference bit-reverses first, root.zig:154-161), so a
+ *     rejected call leaves `data` unchanged.
+ *   - Device-only conditions (HIP errors, device memory, the context's HBM
+ *     budget) are reported as ZNTT_DEVICE_ERROR / ZNTT_OUT_OF_MEMORY; a Zig
+ *     shim that must keep the reference's exact error sets maps them to a
+ *     panic.
+ */
+#ifndef ZNTT_GPU_H
+#define ZNTT_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Status codes (0 = success).                                               */
+/* ------------------------------------------------------------------------ */
+enum zntt_status {
+  ZNTT_OK = 0,
+  ZNTT_LOG_TOO_LARGE = 1,     /* error.LogTooLarge      root.zig:33-36        */
+  ZNTT_LENGTH_MISMATCH = 2,   /* error.LengthMismatch   root.zig:68,98,151,183 */
+  ZNTT_INVALID_LENGTH = 3,    /* error.InvalidLength    root.zig:43           */
+  ZNTT_INVALID_TWIDDLES = 4,  /* error.InvalidTwiddles  root.zig:152,161,184,193 */
+  ZNTT_OUT_OF_MEMORY = 5,     /* error.OutOfMemory      root.zig:112 (and device memory / HBM budget) */
+  ZNTT_DEVICE_ERROR = 6,      /* a HIP runtime call failed                       */
+  ZNTT_NO_DEVICE = 7,         /* zntt_ctx_create: no such HIP device             */
+  ZNTT_INVALID_ARGUMENT = 8,  /* null context/pointer, unknown field id          */
+  ZNTT_UNSUPPORTED = 9,       /* a size this device path does not implement      */
+  ZNTT_ORDER_TOO_LARGE = 10   /* error.OrderTooLarge    roots.zig:150 (LDE/roots) */
+};
+
+enum zntt_field {
+  ZNTT_FIELD_BN254_FR = 1,
+  ZNTT_FIELD_GOLDILOCKS = 2
+};
+
+/* Direction of a device transform (zntt_transform.direction). */
+enum zntt_direction {
+  ZNTT_FORWARD = 0,        /* ntt(root)                                        */
+  ZNTT_INVERSE = 1,        /* intt(root): ntt(root^-1) then scale by n^-1      */
+  ZNTT_COSET_FORWARD = 2   /* data[i] *= shift^i, then ntt(root)               */
+};
+
+/* Largest log2 size the device path implements (2^28 BN254 Fr = 8 GiB). */
+#define ZNTT_MAX_LOG_N 28
+
+typedef struct zntt_ctx zntt_ctx;
+
+/* ------------------------------------------------------------------------ */
+/* Context.  Owns a HIP stream, scratch memory and the device-resident       */
+/* twiddle cache.  No zig-ntt counterpart: zig-ntt is stateless; the shim    */
+/* creates one context per GPU and passes it to every call.                  */
+/* ------------------------------------------------------------------------ */
+
+/* hbm_budget_bytes: cap on device memory the context may hold (0 = 64 GiB). */
+int zntt_ctx_create(zntt_ctx **out, int device, uint64_t hbm_budget_bytes);
+void zntt_ctx_destroy(zntt_ctx *ctx);
+/* Device bytes currently held (scratch + twiddle cache + staging). */
+uint64_t zntt_ctx_bytes_in_use(const zntt_ctx *ctx);
+/* Drops every cached twiddle table and scratch buffer. */
+int zntt_ctx_trim(zntt_ctx *ctx);
+const char *zntt_status_string(int status);
+/* Bytes per element of a field, 0 for an unknown id. */
+size_t zntt_element_size(int field);
+
+/* ------------------------------------------------------------------------ */
+/* The seven zig-ntt functions (host buffers, synchronous, in place).        */
+/* ------------------------------------------------------------------------ */
+
+/* bitReverse(F, data) -- root.zig:38-58 */
+int zntt_bit_reverse(zntt_ctx *ctx, int field, void *data, size_t len);
+
+/* ntt(F, data, log_n, root) -- root.zig:65-90.  `root` points at one element. */
+int zntt_ntt(zntt_ctx *ctx, int field, void *data, size_t len, size_t log_n, const void *root);
+
+/* intt(F, data, log_n, root) -- root.zig:95-107 */
+int zntt_intt(zntt_ctx *ctx, int field, void *data, size_t len, size_t log_n, const void *root);
+
+/* precomputeTwiddles(F, log_n, root, allocator) -- root.zig:112-134.
+ * The caller owns the table memory, as with the reference's allocator: it
+ * passes `num_stages == log_n` host buffers, stage s holding 2^s elements,
+ * and the library fills them (tw[s][j] = (root^(2^(log_n-1-s)))^j).  The
+ * table is also registered, keyed by `stages`, in the context's device-
+ * resident cache so that *_with_twiddles on the same table reads it from HBM.
+ * Errors: LogTooLarge (log_n >= 64), OutOfMemory, InvalidArgument
+ * (num_stages != log_n or a null stage pointer). */
+int zntt_precompute_twiddles(zntt_ctx *ctx, int field, size_t log_n, const void *root,
+                             void *const *stages, size_t num_stages);
+
+/* freeTwiddles(F, tw, allocator) -- root.zig:136-141.  Releases the device
+ * copy registered for `stages`; the caller then frees its host buffers. */
+int zntt_free_twiddles(zntt_ctx *ctx, const void *const *stages, size_t num_stages);
+
+/* nttWithTwiddles(F, data, log_n, twiddles) -- root.zig:148-175.
+ * `stages`/`stage_lens`/`num_stages` are the Zig `[]const []const F`.  A table
+ * registered by zntt_precompute_twiddles is read from the device cache; any
+ * other (foreign) table is uploaded and used exactly as given. */
+int zntt_ntt_with_twiddles(zntt_ctx *ctx, int field, void *data, size_t len, size_t log_n,
+                           const void *const *stages, const size_t *stage_lens, size_t num_stages);
+
+/* inttWithTwiddles(F, data, log_n, twiddles) -- root.zig:180-212
+ * (w = 1 for j = 0, else -tw[s-1][h-j]; then scale by n^-1). */
+int zntt_intt_with_twiddles(zntt_ctx *ctx, int field, void *data, size_t len, size_t log_n,
+                            const void *const *stages, const size_t *stage_lens, size_t num_stages);
+
+/* ------------------------------------------------------------------------ */
+/* Beyond the zig-ntt surface.  No reference counterpart: the reference has  */
+/* no coset transform, LDE, device-resident or batched API.                  */
+/* ------------------------------------------------------------------------ */
+
+/* Coset-forward NTT (host buffer): data[i] *= shift^i, then ntt(root).
+ * Errors as zntt_ntt. */
+int zntt_coset_ntt(zntt_ctx *ctx, int field, void *data, size_t len, size_t log_n,
+                   const void *root, const void *shift);
+
+/* Low-degree extension (host buffers): zero-pad 2^log_k coefficients to
+ * 2^(log_k+log_blowup) and run the coset-forward NTT with the field's
+ * primitive 2^(log_k+log_blowup)-th root (libs/field/src/roots.zig:149-157).
+ * `out` holds 2^(log_k+log_blowup) elements. */
+int zntt_lde(zntt_ctx *ctx, int field, const void *coeffs, size_t log_k, size_t log_blowup,
+             const void *shift, void *out);
+
+/* The field's primitive 2^log_size-th root of unity, as
+ * libs/field/src/roots.zig:149-157 defines it (z = smallest QNR >= 2,
+ * z^(odd_part << (two_adicity - log_size))).  ORDER_TOO_LARGE past the
+ * field's two-adicity. */
+int zntt_primitive_root(int field, size_t log_size, void *out);
+
+/* One transform on device-resident data (HBM), enqueued on `stream`
+ * (a hipStream_t; NULL = the context's own stream).  Asynchronous: returns
+ * once the work is enqueued.  `root`/`shift` are host pointers to one
+ * element (shift only for ZNTT_COSET_FORWARD).  The twiddle table comes from
+ * the device-resident cache (built on first use, on `stream`). */
+typedef struct {
+  int field;           /* zntt_field                          */
+  int direction;       /* zntt_direction                      */
+  size_t log_n;        /* transform size 2^log_n              */
+  void *data;          /* device pointer, 2^log_n elements    */
+  const void *root;    /* host pointer, one element           */
+  const void *shift;   /* host pointer, coset shift (or NULL) */
+} zntt_transform;
+
+int zntt_execute_device(zntt_ctx *ctx, const zntt_transform *t, void *stream);
+
+/* Blocks until every call enqueued on the context's own stream is done. */
+int zntt_ctx_synchronize(zntt_ctx *ctx);
+
+/* ------------------------------------------------------------------------ */
+/* Diagnostics for tests and the benchmark.                                  */
+/* ------------------------------------------------------------------------ */
+
+/* Elementwise field operation on device: out[i] = op(a[i], b[i]) for the
+ * library's device arithmetic (op 0 add, 1 sub, 2 mul), host buffers of n
+ * elements in the field's storage layout.  Used to check the device field
+ * code against an independent implementation. */
+int zntt_debug_field_op(zntt_ctx *ctx, int field, int op, const void *a, const void *b,
+                        void *out, size_t n);
+
+/* Seeded uniform elements in the field's storage layout (host memory): a
+ * SplitMix64-seeded xoshiro256++ stream; each draw takes 32 (Fr) or 8
+ * (Goldilocks) little-endian bytes and is rejected if >= the modulus, as the
+ * reference's `random` does (libs/field/src/field.zig:982-990, :224-232); Fr
+ * values are then put in Montgomery form.  The benchmark's input generator. */
+int zntt_fill_uniform(int field, uint64_t seed, void *out, size_t n);
+
+/* Measured rate of the device's v_mad_u64_u32 (32x32->64 multiply-add), in
+ * multiplies per second, from an unrolled independent-chain microbenchmark;
+ * and the measured device-to-device copy bandwidth in bytes per second. */
+int zntt_probe_peaks(zntt_ctx *ctx, double *mad_per_s, double *copy_bytes_per_s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ZNTT_GPU_H */

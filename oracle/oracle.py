@@ -1,0 +1,229 @@
+"""ctypes loader for the CPU oracle (``oracle/liboracle.so``).
+
+TEST INFRASTRUCTURE ONLY: imported by ``tests/``, ``__graft_entry__.smoke()``
+and the ``cpu_baseline`` leg of ``bench.py`` -- as the checker, never as the
+thing measured or shipped.  The oracle restates
+``libs/ntt/src/root.zig:33-212`` over ``libs/field/src/montgomery.zig`` (BN254
+Fr) and ``libs/field/src/field.zig:89-424`` (small fields); see the header of
+``zntt_oracle.c`` for the line-by-line map.
+
+Element layout (identical to the reference's in-memory Zig types):
+  * BN254 Fr: numpy ``uint64`` array of shape ``(n, 4)`` -- four little-endian
+    limbs of the Montgomery form ``x * 2^256 mod r``;
+  * Goldilocks / BabyBear: numpy ``uint64`` array of shape ``(n,)`` holding the
+    canonical value.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+FIELD_FR = 1
+FIELD_GL = 2
+FIELD_BB = 3
+
+FR_MODULUS = 0x30644E72E131A029B85045B68181585D2833E84879B9709143E1F593F0000001
+GL_MODULUS = 0xFFFFFFFF00000001
+BB_MODULUS = 0x78000001
+MODULUS = {FIELD_FR: FR_MODULUS, FIELD_GL: GL_MODULUS, FIELD_BB: BB_MODULUS}
+
+OK = 0
+LOG_TOO_LARGE = 1
+LENGTH_MISMATCH = 2
+INVALID_LENGTH = 3
+INVALID_TWIDDLES = 4
+OUT_OF_MEMORY = 5
+INVALID_ARGUMENT = 8
+ORDER_TOO_LARGE = 10
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle with its Makefile (gcc, no GPU needed)."""
+    args = ["make", "-C", _HERE]
+    if force:
+        args.append("-B")
+    subprocess.run(args, check=True, stdout=subprocess.DEVNULL)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        vp, sz, u64, i32 = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint64, ctypes.c_int
+        L.oracle_init.restype = None
+        L.oracle_elem_size.argtypes = [i32]
+        L.oracle_field_op.argtypes = [i32, i32, vp, vp, vp]
+        L.oracle_primitive_root.argtypes = [i32, sz, vp]
+        L.oracle_bit_reverse.argtypes = [i32, vp, sz]
+        L.oracle_ntt.argtypes = [i32, vp, sz, sz, vp]
+        L.oracle_intt.argtypes = [i32, vp, sz, sz, vp]
+        L.oracle_coset_ntt.argtypes = [i32, vp, sz, sz, vp, vp]
+        L.oracle_precompute_twiddles.argtypes = [i32, sz, vp, vp]
+        L.oracle_ntt_with_twiddles.argtypes = [i32, i32, vp, sz, sz, vp, vp, sz]
+        L.oracle_lde.argtypes = [i32, vp, sz, sz, vp, vp]
+        L.oracle_gen_uniform.argtypes = [i32, u64, vp, sz]
+        L.oracle_xoshiro_stream.argtypes = [u64, vp, sz]
+        L.oracle_xoshiro_stream.restype = None
+        L.oracle_ntt_batch_mt.argtypes = [i32, vp, vp, vp, vp, sz, i32]
+        L.oracle_init()
+        _lib = L
+    return _lib
+
+
+def limbs(field: int) -> int:
+    return 4 if field == FIELD_FR else 1
+
+
+def empty(field: int, n: int) -> np.ndarray:
+    return np.zeros((n, 4) if field == FIELD_FR else (n,), dtype=np.uint64)
+
+
+def _ptr(a: np.ndarray | None):
+    if a is None:
+        return None
+    assert isinstance(a, np.ndarray) and a.flags["C_CONTIGUOUS"] and a.dtype == np.uint64
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _as_elem(field: int, x) -> np.ndarray:
+    """One element as a contiguous 1-D uint64 array of its limbs."""
+    return np.ascontiguousarray(np.asarray(x, dtype=np.uint64).reshape(-1))
+
+
+def elem(field: int, value: int) -> np.ndarray:
+    """Raw element bytes from an integer already in storage form."""
+    if field == FIELD_FR:
+        return np.array([(value >> (64 * i)) & (2**64 - 1) for i in range(4)], dtype=np.uint64)
+    return np.array([value], dtype=np.uint64)
+
+
+def to_int(field: int, a: np.ndarray) -> int:
+    a = np.asarray(a, dtype=np.uint64).reshape(-1)
+    return sum(int(x) << (64 * i) for i, x in enumerate(a))
+
+
+# --- field arithmetic ------------------------------------------------------
+_OPS = {"add": 0, "sub": 1, "mul": 2, "neg": 3, "inv": 4, "to_mont": 5, "from_mont": 6, "pow": 7}
+
+
+def field_op(field: int, op: str, a, b=None):
+    """One field operation; returns the same shape as ``a`` (a scalar for a
+    small-field scalar, four limbs for Fr)."""
+    scalar = np.ndim(a) == 0
+    ea = _as_elem(field, a)
+    eb = None if b is None else _as_elem(field, b)
+    out = np.zeros_like(ea)
+    st = lib().oracle_field_op(field, _OPS[op], _ptr(ea), _ptr(eb), _ptr(out))
+    assert st == OK, st
+    return out[0] if scalar else out
+
+
+def from_int(field: int, x: int) -> np.ndarray:
+    """Element for integer x (reduced mod p); Montgomery form for Fr."""
+    x %= MODULUS[field]
+    e = elem(field, x)
+    return field_op(field, "to_mont", e) if field == FIELD_FR else e
+
+
+def value(field: int, a) -> int:
+    """Canonical integer of one element."""
+    return to_int(field, field_op(field, "from_mont", _as_elem(field, a)) if field == FIELD_FR else a)
+
+
+def from_ints(field: int, ints) -> np.ndarray:
+    """Storage-form array from canonical integers."""
+    a = empty(field, len(ints))
+    for i, x in enumerate(ints):
+        e = from_int(field, x)
+        a[i] = e if field == FIELD_FR else e[0]
+    return a
+
+
+def primitive_root(field: int, log_size: int) -> tuple[int, np.ndarray]:
+    out = empty(field, 1).reshape(-1)
+    st = lib().oracle_primitive_root(field, log_size, _ptr(out))
+    return st, out
+
+
+# --- the seven functions + coset/LDE ---------------------------------------
+def bit_reverse(field: int, data: np.ndarray) -> int:
+    return lib().oracle_bit_reverse(field, _ptr(data), data.shape[0])
+
+
+def ntt(field: int, data: np.ndarray, log_n: int, root: np.ndarray, length: int | None = None) -> int:
+    n = data.shape[0] if length is None else length
+    return lib().oracle_ntt(field, _ptr(data), n, log_n, _ptr(_as_elem(field, root)))
+
+
+def intt(field: int, data: np.ndarray, log_n: int, root: np.ndarray, length: int | None = None) -> int:
+    n = data.shape[0] if length is None else length
+    return lib().oracle_intt(field, _ptr(data), n, log_n, _ptr(_as_elem(field, root)))
+
+
+def coset_ntt(field: int, data: np.ndarray, log_n: int, root: np.ndarray, shift: np.ndarray) -> int:
+    return lib().oracle_coset_ntt(field, _ptr(data), data.shape[0], log_n, _ptr(_as_elem(field, root)),
+                                  _ptr(_as_elem(field, shift)))
+
+
+def precompute_twiddles(field: int, log_n: int, root: np.ndarray) -> tuple[int, list[np.ndarray]]:
+    """Returns (status, stages); stage s holds 2^s elements."""
+    if log_n >= 64:
+        flat = empty(field, 1)
+        st = lib().oracle_precompute_twiddles(field, log_n, _ptr(_as_elem(field, root)), _ptr(flat))
+        return st, []
+    total = (1 << log_n) - 1
+    flat = empty(field, max(total, 1))
+    st = lib().oracle_precompute_twiddles(field, log_n, _ptr(_as_elem(field, root)), _ptr(flat))
+    stages = [flat[(1 << s) - 1:(1 << (s + 1)) - 1] for s in range(log_n)]
+    return st, stages
+
+
+def ntt_with_twiddles(field: int, data: np.ndarray, log_n: int, stages: list[np.ndarray],
+                      inverse: bool = False) -> int:
+    stages = [np.ascontiguousarray(s) for s in stages]
+    ptrs = (ctypes.c_void_p * max(len(stages), 1))(*[s.ctypes.data for s in stages])
+    lens = (ctypes.c_size_t * max(len(stages), 1))(*[s.shape[0] for s in stages])
+    return lib().oracle_ntt_with_twiddles(field, int(inverse), _ptr(data), data.shape[0], log_n,
+                                          ctypes.cast(ptrs, ctypes.c_void_p),
+                                          ctypes.cast(lens, ctypes.c_void_p), len(stages))
+
+
+def intt_with_twiddles(field: int, data: np.ndarray, log_n: int, stages: list[np.ndarray]) -> int:
+    return ntt_with_twiddles(field, data, log_n, stages, inverse=True)
+
+
+def lde(field: int, coeffs: np.ndarray, log_k: int, log_blowup: int, shift: np.ndarray) -> tuple[int, np.ndarray]:
+    out = empty(field, 1 << (log_k + log_blowup))
+    st = lib().oracle_lde(field, _ptr(coeffs), log_k, log_blowup, _ptr(_as_elem(field, shift)), _ptr(out))
+    return st, out
+
+
+def gen_uniform(field: int, seed: int, n: int) -> np.ndarray:
+    out = empty(field, n)
+    st = lib().oracle_gen_uniform(field, seed, _ptr(out), n)
+    assert st == OK
+    return out
+
+
+def xoshiro_stream(seed: int, n: int) -> np.ndarray:
+    out = np.zeros(n, dtype=np.uint64)
+    lib().oracle_xoshiro_stream(seed, _ptr(out), n)
+    return out
+
+
+def ntt_batch_mt(field: int, data: np.ndarray, offsets: np.ndarray, log_ns: np.ndarray,
+                 roots_by_log: np.ndarray, threads: int) -> int:
+    offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
+    log_ns = np.ascontiguousarray(log_ns, dtype=np.uint64)
+    return lib().oracle_ntt_batch_mt(field, _ptr(data), _ptr(offsets), _ptr(log_ns),
+                                     _ptr(roots_by_log), len(log_ns), threads)

@@ -1,0 +1,93 @@
+"""CPU-side checks of the C ABI library (no GPU compute).
+
+* libzntt_gpu.so loads and exports every function include/zntt_gpu.h declares;
+* context creation without a GPU fails with ZNTT_NO_DEVICE instead of crashing;
+* host-only entry points (status strings, element sizes, primitive roots)
+  agree with the Python-int fixtures;
+* the C oracle's status codes equal the ABI's.
+"""
+import json
+import os
+import re
+
+import pytest
+
+import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "zntt_gpu.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(zntt_[a-z0-9_]+)\s*\(", text)))
+
+
+@pytest.fixture(scope="module")
+def Z():
+    import zntt_amd
+    return zntt_amd
+
+
+def test_library_exports_every_declared_symbol(Z):
+    decl = declared_functions()
+    assert len(decl) >= 20
+    for name in decl:
+        assert hasattr(Z.lib, name), name
+    assert sorted(Z.EXPORTED) == decl
+
+
+def test_status_codes_match_oracle_and_reference_error_names(Z):
+    assert Z.LogTooLarge.status == O.LOG_TOO_LARGE
+    assert Z.LengthMismatch.status == O.LENGTH_MISMATCH
+    assert Z.InvalidLength.status == O.INVALID_LENGTH
+    assert Z.InvalidTwiddles.status == O.INVALID_TWIDDLES
+    assert Z.OutOfMemory.status == O.OUT_OF_MEMORY
+    assert Z.OrderTooLarge.status == O.ORDER_TOO_LARGE
+    for cls in (Z.LogTooLarge, Z.LengthMismatch, Z.InvalidLength, Z.InvalidTwiddles, Z.OutOfMemory,
+                Z.OrderTooLarge):
+        assert Z.lib.zntt_status_string(cls.status).decode() == cls.__name__
+
+
+def test_element_sizes(Z):
+    assert Z.element_size(Z.FIELD_FR) == 32
+    assert Z.element_size(Z.FIELD_GL) == 8
+    assert Z.element_size(99) == 0
+
+
+def test_primitive_roots_host_side(Z):
+    c = json.load(open(os.path.join(ROOT, "tests", "golden", "fr_constants.json")))
+    for l in (0, 1, 5, 16, 20, 24, 28):
+        r = Z.primitive_root(Z.FIELD_FR, l)
+        assert O.value(O.FIELD_FR, r) == int(c["primitive_roots"][str(l)], 16)
+    for l in (0, 1, 12, 32):
+        r = Z.primitive_root(Z.FIELD_GL, l)
+        assert int(r[0]) == int(c["goldilocks_primitive_roots"][str(l)], 16)
+    with pytest.raises(Z.OrderTooLarge):
+        Z.primitive_root(Z.FIELD_FR, 29)
+    with pytest.raises(Z.OrderTooLarge):
+        Z.primitive_root(Z.FIELD_GL, 33)
+
+
+def test_context_without_gpu_reports_no_device(Z):
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    with pytest.raises(Z.NoDevice):
+        Z.Context(0)
+
+
+def test_library_generator_matches_oracle(Z):
+    """The benchmark's input generator (host code in the library) draws the
+    same stream as the oracle's and the Python-int restatement."""
+    import numpy as np
+    import pyref
+    for F in (Z.FIELD_FR, Z.FIELD_GL):
+        got = Z.fill_uniform(F, 3, 500)
+        assert np.array_equal(got, O.gen_uniform(F, 3, 500))
+    c = json.load(open(os.path.join(ROOT, "tests", "golden", "textbook_digests.json")))["cases"]["fr_2p16_seed0"]
+    import hashlib
+    assert hashlib.sha256(Z.fill_uniform(Z.FIELD_FR, 0, 1 << 16).tobytes()).hexdigest() == c["input_digest"]
+    assert [O.value(O.FIELD_FR, Z.fill_uniform(Z.FIELD_FR, 0, 4)[i]) for i in range(4)] == \
+        pyref.uniform_elements(pyref.FR, 0, 4)

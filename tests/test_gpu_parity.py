@@ -1,0 +1,326 @@
+"""GPU parity: the HIP library through its C ABI against the CPU oracle.
+
+Bit-exact comparison (the path is integer arithmetic) on seeded inputs at
+sizes the oracle finishes in seconds, against committed digests at the full
+workload sizes (BN254 Fr 2^20 / 2^22 / 2^24), and through round trips.  Edge
+cases follow the reference's tests (libs/ntt/src/root.zig:290-738): sizes 2^0
+and 2^1, every pass split, error contracts, non-primitive roots and foreign
+(non-power) twiddle tables, which only the exact DIT DAG reproduces.
+"""
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+import pyref
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+FIELDS = [O.FIELD_FR, O.FIELD_GL]
+FNAME = {O.FIELD_FR: "fr", O.FIELD_GL: "gl"}
+
+
+def load(name):
+    with open(os.path.join(GOLD, name)) as f:
+        return json.load(f)
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+@pytest.fixture(scope="module")
+def Z():
+    import zntt_amd
+    return zntt_amd
+
+
+@pytest.fixture(scope="module")
+def ctx(Z):
+    c = Z.Context(0)
+    yield c
+    c.close()
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+# ---------------------------------------------------------------------------
+# Device field arithmetic
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("field", FIELDS, ids=["fr", "gl"])
+def test_device_field_ops_match_oracle(Z, ctx, field):
+    say("field ops", FNAME[field])
+    n = 4096
+    a = O.gen_uniform(field, 11, n)
+    b = O.gen_uniform(field, 12, n)
+    p = O.MODULUS[field]
+    edge = [0, 1, 2, p - 1, p - 2, (p - 1) // 2]
+    ea = O.from_ints(field, [x for x in edge for _ in edge])
+    eb = O.from_ints(field, [y for _ in edge for y in edge])
+    a = np.concatenate([a, ea])
+    b = np.concatenate([b, eb])
+    for op, name in ((0, "add"), (1, "sub"), (2, "mul")):
+        got = ctx.debug_field_op(field, op, a, b)
+        exp = np.stack([O.field_op(field, name, a[i], b[i]) for i in range(a.shape[0])])
+        assert np.array_equal(got.reshape(exp.shape), exp), name
+
+
+# ---------------------------------------------------------------------------
+# ntt / intt against the oracle at every pass split
+# ---------------------------------------------------------------------------
+
+SIZES = list(range(0, 15)) + [16, 17, 19]
+
+
+@pytest.mark.parametrize("field", FIELDS, ids=["fr", "gl"])
+@pytest.mark.parametrize("log_n", SIZES)
+def test_ntt_intt_match_oracle(Z, ctx, field, log_n):
+    say("ntt/intt", FNAME[field], log_n)
+    n = 1 << log_n
+    x = O.gen_uniform(field, 1000 + log_n, n)
+    root = O.primitive_root(field, log_n)[1]
+    exp = x.copy()
+    assert O.ntt(field, exp, log_n, root) == O.OK
+    got = x.copy()
+    ctx.ntt(field, got, log_n, root)
+    assert np.array_equal(got, exp)
+    exp_i = x.copy()
+    assert O.intt(field, exp_i, log_n, root) == O.OK
+    got_i = x.copy()
+    ctx.intt(field, got_i, log_n, root)
+    assert np.array_equal(got_i, exp_i)
+    ctx.intt(field, got, log_n, root)
+    assert np.array_equal(got, x)
+
+
+def test_reference_known_answers_on_device(Z, ctx):
+    """libs/ntt/src/root.zig:438-443 (the Goldilocks rows) through the GPU."""
+    for c in load("ref_ntt_kat.json")["cases"]:
+        if c["field"] != "goldilocks":
+            continue
+        xs = np.array([int(v, 16) for v in c["input"]], dtype=np.uint64)
+        root = np.array([int(c["root"], 16)], dtype=np.uint64)
+        data = xs.copy()
+        ctx.ntt(Z.FIELD_GL, data, c["log_n"], root)
+        assert [int(v) for v in data] == [int(v, 16) for v in c["expect"]]
+        ctx.intt(Z.FIELD_GL, data, c["log_n"], root)
+        assert np.array_equal(data, xs)
+
+
+def test_fr_dft_vectors_on_device(Z, ctx):
+    for c in load("dft_vectors.json")["dft"]:
+        F = {"fr": O.FIELD_FR, "goldilocks": O.FIELD_GL}.get(c["field"])
+        if F is None:
+            continue
+        data = O.from_ints(F, [int(v, 16) for v in c["input"]])
+        root = O.from_int(F, int(c["root"], 16))
+        ctx.ntt(F, data, c["log_n"], root)
+        assert [O.value(F, data[i]) for i in range(data.shape[0])] == [int(v, 16) for v in c["dft"]]
+
+
+@pytest.mark.parametrize("field", FIELDS, ids=["fr", "gl"])
+def test_non_primitive_root_follows_the_dag(Z, ctx, field):
+    """For a root that is not a primitive root the reference's DIT DAG is the
+    only definition; the device must reproduce it exactly."""
+    for log_n in (5, 12, 15):
+        x = O.gen_uniform(field, 77 + log_n, 1 << log_n)
+        root = O.from_int(field, 123456789)
+        exp = x.copy()
+        O.ntt(field, exp, log_n, root)
+        got = x.copy()
+        ctx.ntt(field, got, log_n, root)
+        assert np.array_equal(got, exp), log_n
+
+
+# ---------------------------------------------------------------------------
+# Twiddled variants
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("field", FIELDS, ids=["fr", "gl"])
+@pytest.mark.parametrize("log_n", [1, 3, 10, 13, 16])
+def test_precompute_and_twiddled_transforms(Z, ctx, field, log_n):
+    say("twiddled", FNAME[field], log_n)
+    root = O.primitive_root(field, log_n)[1]
+    tw = ctx.precompute_twiddles(field, log_n, root)
+    st, otw = O.precompute_twiddles(field, log_n, root)
+    assert st == O.OK and len(tw) == log_n
+    for s in range(log_n):
+        assert np.array_equal(tw[s], otw[s]), s
+    x = O.gen_uniform(field, 5000 + log_n, 1 << log_n)
+    exp = x.copy()
+    O.ntt_with_twiddles(field, exp, log_n, otw)
+    got = x.copy()
+    ctx.ntt_with_twiddles(field, got, log_n, tw)
+    assert np.array_equal(got, exp)
+    exp_i = exp.copy()
+    O.intt_with_twiddles(field, exp_i, log_n, otw)
+    ctx.intt_with_twiddles(field, got, log_n, tw)
+    assert np.array_equal(got, exp_i)
+    assert np.array_equal(got, x)
+    tw.free()
+
+
+@pytest.mark.parametrize("field", FIELDS, ids=["fr", "gl"])
+def test_foreign_non_power_tables(Z, ctx, field):
+    """A caller's table that is not powers of anything (root.zig:148-212 uses
+    it as given): forward and the -tw[h-j] inverse must match the oracle."""
+    for log_n in (4, 12, 14):
+        stages = [O.gen_uniform(field, 900 + s, 1 << s) for s in range(log_n)]
+        x = O.gen_uniform(field, 800 + log_n, 1 << log_n)
+        for inverse in (False, True):
+            exp = x.copy()
+            O.ntt_with_twiddles(field, exp, log_n, stages, inverse=inverse)
+            got = x.copy()
+            (ctx.intt_with_twiddles if inverse else ctx.ntt_with_twiddles)(field, got, log_n, stages)
+            assert np.array_equal(got, exp), (log_n, inverse)
+
+
+# ---------------------------------------------------------------------------
+# Error contracts (libs/ntt/src/root.zig:675-738)
+# ---------------------------------------------------------------------------
+
+def test_error_contracts(Z, ctx):
+    for F in FIELDS:
+        root = O.from_int(F, 3)
+        short = O.empty(F, 3)
+        with pytest.raises(Z.LengthMismatch):
+            ctx.ntt(F, short, 2, root)
+        with pytest.raises(Z.LengthMismatch):
+            ctx.intt(F, short, 2, root)
+        two = O.empty(F, 2)
+        with pytest.raises(Z.LogTooLarge):
+            ctx.ntt(F, two, 64, root)
+        with pytest.raises(Z.LogTooLarge):
+            ctx.intt(F, two, 200, root)
+        with pytest.raises(Z.LogTooLarge):
+            ctx.precompute_twiddles(F, 64, root)
+        with pytest.raises(Z.InvalidLength):
+            ctx.bit_reverse(F, O.empty(F, 0))
+        with pytest.raises(Z.InvalidLength):
+            ctx.bit_reverse(F, O.empty(F, 3))
+    F = O.FIELD_GL
+    log_n = 3
+    root = O.primitive_root(F, log_n)[1]
+    tw = ctx.precompute_twiddles(F, log_n, root)
+    data = O.from_ints(F, [1] * 8)
+    before = data.copy()
+    with pytest.raises(Z.InvalidTwiddles):
+        ctx.ntt_with_twiddles(F, data, log_n, tw.stages[:2])
+    with pytest.raises(Z.InvalidTwiddles):
+        ctx.intt_with_twiddles(F, data, log_n, tw.stages[:2])
+    with pytest.raises(Z.InvalidTwiddles):
+        ctx.ntt_with_twiddles(F, data, log_n, [tw[0][:1], tw[1][:1], tw[2]])
+    assert np.array_equal(data, before)  # rejected before the data is touched
+    copy = data.copy()
+    ctx.ntt_with_twiddles(F, copy, log_n, tw)
+    ctx.intt_with_twiddles(F, copy, log_n, tw)
+    assert np.array_equal(copy, data)
+    tw.free()
+
+
+@pytest.mark.parametrize("field", FIELDS, ids=["fr", "gl"])
+def test_bit_reverse_matches_oracle(Z, ctx, field):
+    for log_n in (0, 1, 2, 7, 13):
+        x = O.gen_uniform(field, 60 + log_n, 1 << log_n)
+        exp = x.copy()
+        O.bit_reverse(field, exp)
+        got = x.copy()
+        ctx.bit_reverse(field, got)
+        assert np.array_equal(got, exp)
+
+
+# ---------------------------------------------------------------------------
+# Coset and LDE
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("field", FIELDS, ids=["fr", "gl"])
+def test_coset_and_lde_match_oracle_and_horner(Z, ctx, field):
+    shift = O.from_int(field, 5 if field == O.FIELD_FR else 7)
+    for log_n in (0, 1, 6, 12, 15):
+        x = O.gen_uniform(field, 300 + log_n, 1 << log_n)
+        root = O.primitive_root(field, log_n)[1]
+        exp = x.copy()
+        O.coset_ntt(field, exp, log_n, root, shift)
+        got = x.copy()
+        ctx.coset_ntt(field, got, log_n, root, shift)
+        assert np.array_equal(got, exp), log_n
+    fname = "fr" if field == O.FIELD_FR else "goldilocks"
+    for c in load("dft_vectors.json")["lde"]:
+        if c["field"] != fname:
+            continue
+        coeffs = O.from_ints(field, [int(v, 16) for v in c["coeffs"]])
+        out = ctx.lde(field, coeffs, c["log_k"], c["log_blowup"], O.from_int(field, int(c["coset_shift"], 16)))
+        assert [O.value(field, out[i]) for i in range(out.shape[0])] == [int(v, 16) for v in c["evals"]]
+    for log_k, log_b in ((10, 2), (12, 3)):
+        coeffs = O.gen_uniform(field, 40 + log_k, 1 << log_k)
+        st, exp = O.lde(field, coeffs, log_k, log_b, shift)
+        assert st == O.OK
+        got = ctx.lde(field, coeffs, log_k, log_b, shift)
+        assert np.array_equal(got, exp)
+
+
+# ---------------------------------------------------------------------------
+# Full-size workloads (BN254 Fr 2^16 .. 2^24) against committed digests
+# ---------------------------------------------------------------------------
+
+def test_c1_roundtrip_2p16_seed0(Z, ctx):
+    say("C1 2^16")
+    c = load("textbook_digests.json")["cases"]["fr_2p16_seed0"]
+    F = O.FIELD_FR
+    x = O.gen_uniform(F, 0, 1 << 16)
+    assert sha(x) == c["input_digest"]
+    root = Z.primitive_root(Z.FIELD_FR, 16)
+    data = x.copy()
+    ctx.ntt(F, data, 16, root)
+    assert sha(data) == c["forward_digest"]
+    ctx.intt(F, data, 16, root)
+    assert np.array_equal(data, x)
+
+
+@pytest.mark.parametrize("log_n", [20, 22, 24])
+def test_c2_large_fr_against_oracle_digests(Z, ctx, log_n):
+    say("C2", log_n)
+    d = load("oracle_digests.json")["fr"][str(log_n)]
+    F = O.FIELD_FR
+    x = O.gen_uniform(F, 1, 1 << log_n)
+    assert sha(x) == d["input"]
+    root = Z.primitive_root(Z.FIELD_FR, log_n)
+    fwd = x.copy()
+    ctx.ntt(F, fwd, log_n, root)
+    assert sha(fwd) == d["forward"]
+    if log_n == 20:
+        assert sha(fwd) == load("textbook_digests.json")["cases"]["fr_2p20_seed1"]["forward_digest"]
+    inv = x.copy()
+    ctx.intt(F, inv, log_n, root)
+    assert sha(inv) == d["inverse"]
+    ctx.intt(F, fwd, log_n, root)  # round trip
+    assert np.array_equal(fwd, x)
+
+
+def test_device_resident_execute(Z, ctx):
+    """zntt_execute_device on torch-allocated HBM, on torch's stream."""
+    import torch
+    F = O.FIELD_FR
+    log_n = 18
+    x = O.gen_uniform(F, 9, 1 << log_n)
+    root = O.primitive_root(F, log_n)[1]
+    shift = O.from_int(F, 5)
+    t = torch.from_numpy(x.view(np.int64).copy()).cuda()
+    s = torch.cuda.current_stream().cuda_stream
+    for direction, fn in ((Z.FORWARD, O.ntt), (Z.INVERSE, O.intt), (Z.COSET_FORWARD, None)):
+        d = t.clone()
+        ctx.execute_device(F, direction, log_n, d.data_ptr(), root, shift if fn is None else None, stream=s)
+        torch.cuda.synchronize()
+        exp = x.copy()
+        if fn is None:
+            O.coset_ntt(F, exp, log_n, root, shift)
+        else:
+            fn(F, exp, log_n, root)
+        assert np.array_equal(d.cpu().numpy().view(np.uint64).reshape(exp.shape), exp), direction

@@ -1,0 +1,210 @@
+// BN254 scalar field r on gfx950 VALU, radix 2^29.
+//
+// Storage (user) format: the reference's in-memory `Fr` -- four little-endian
+// u64 limbs of the Montgomery form x * 2^256 mod r (libs/field/src/field.zig:884,
+// libs/field/src/montgomery.zig:23-79).  That is byte-for-byte eight
+// little-endian u32 words, which is how the kernels read and write it.
+//
+// Register format: nine 29-bit limbs.  On gfx950 v_mad_u64_u32 issues at the
+// same rate as a 32-bit add, so the cost of a modular multiply is its total
+// VALU instruction count.  With 29-bit limbs a column of the product-scanning
+// Montgomery multiply sums at most 9 products a_i*b_j plus 9 products m_i*r_j
+// and a carry, which stays below 2^64: every partial product is one
+// accumulate-only v_mad_u64_u32 with no carry-out handling.  (With 32-bit
+// limbs each product needs a second instruction to catch its carry.)
+//
+// Montgomery radix: R' = 2^261 (9 * 29).  Data stays in the reference's
+// 2^256 form throughout; only constants -- twiddles, coset powers, scale
+// factors -- are kept in the 2^261 form ("dev form", c * 2^261 mod r).  Then
+//     mont(x * 2^256, c * 2^261) = x * c * 2^256 * 2^261 / 2^261 = x * c * 2^256,
+// so a product of a datum and a constant is again in the reference's form.
+//
+// Lazy reduction.  Values are not reduced modulo r between butterflies:
+//   * mont(a, b) < a*b/2^261 + r < 2r whenever a < 2^261 and b < r;
+//   * the butterfly writes u + t and u + 4r - t, so a transform of 2^L points
+//     keeps every value below r + 4r*L < 2^261 for L <= 28;
+//   * limbs of the sum/difference may exceed 29 bits (up to 1.5 * 2^30 after
+//     one stage); mont() accepts a-limbs below 3 * 2^30 (the column bound), so
+//     only the `u` input of each butterfly is carry-normalised first.
+// The last pass reduces each value fully (quotient estimate from the top limb,
+// then conditional subtractions) and repacks it into the 8-word layout, so
+// results are canonical and bit-identical to the reference's.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace zntt {
+
+constexpr uint32_t kM29 = 0x1FFFFFFFu;
+
+// r in radix 2^29.
+#define ZNTT_FR_P29 {0x10000001u, 0x1f0fac9fu, 0x0e5c2450u, 0x07d090f3u, 0x1585d283u, \
+                     0x02db40c0u, 0x00a6e141u, 0x0e5c2634u, 0x0030644eu}
+// 4r in a redundant radix-2^29 form whose limbs 0..7 lie in [2^29, 2^30), so
+// that u + 4r - t never borrows limb-wise when t is carry-normalised.
+#define ZNTT_FR_C4R {0x20000004u, 0x3c3eb27du, 0x39709142u, 0x3f4243ccu, 0x36174a0bu, \
+                     0x2b6d0301u, 0x229b8503u, 0x397098cfu, 0x00c19138u}
+// -r^-1 mod 2^29
+constexpr uint32_t kFrN0 = 0x0FFFFFFFu;
+// floor(2^40 / (r_8 + 1)) with r_8 = r >> 232 = 0x30644e, for the quotient estimate.
+constexpr uint32_t kFrQMagic = 346694u;
+
+struct Fr9 {
+  uint32_t v[9];
+};
+
+// Hides a constant from the optimiser so the multiply by r_0 = 2^28 + 1 stays a
+// single v_mad_u64_u32 instead of being strength-reduced into two 64-bit ops.
+__device__ __forceinline__ uint32_t fr_opaque(uint32_t x) {
+  asm volatile("" : "+s"(x));
+  return x;
+}
+
+// Montgomery product a * b * 2^-261 mod r (result < 2r, carry-normalised).
+// a: limbs < 3 * 2^30, value < 2^261.   b: carry-normalised, value < r.
+__device__ __forceinline__ Fr9 fr_mul(const Fr9 &a, const Fr9 &b) {
+  constexpr uint32_t P[9] = ZNTT_FR_P29;
+  const uint32_t p0 = fr_opaque(P[0]);
+  uint32_t m[9];
+  uint64_t acc = 0;
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+#pragma unroll
+    for (int i = 0; i <= k; i++) acc += (uint64_t)a.v[i] * b.v[k - i];
+#pragma unroll
+    for (int i = 0; i < k; i++) acc += (uint64_t)m[i] * P[k - i];
+    m[k] = ((uint32_t)acc * kFrN0) & kM29;
+    acc += (uint64_t)m[k] * p0;
+    acc >>= 29;
+  }
+  Fr9 o;
+#pragma unroll
+  for (int k = 9; k < 17; k++) {
+#pragma unroll
+    for (int i = k - 8; i < 9; i++) {
+      acc += (uint64_t)a.v[i] * b.v[k - i];
+      acc += (uint64_t)m[i] * P[k - i];
+    }
+    o.v[k - 9] = (uint32_t)acc & kM29;
+    acc >>= 29;
+  }
+  o.v[8] = (uint32_t)acc;
+  return o;
+}
+
+// Carry-propagate limbs 0..7 into 29 bits; the top limb absorbs the rest.
+__device__ __forceinline__ void fr_norm(Fr9 &x) {
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    x.v[i + 1] += x.v[i] >> 29;
+    x.v[i] &= kM29;
+  }
+}
+
+// Radix-2 DIT butterfly with lazy reduction: (u, v) <- (u + w v, u - w v).
+__device__ __forceinline__ void fr_butterfly(Fr9 &u, Fr9 &v, const Fr9 &w) {
+  constexpr uint32_t C[9] = ZNTT_FR_C4R;
+  Fr9 t = fr_mul(v, w);
+  fr_norm(u);
+#pragma unroll
+  for (int i = 0; i < 9; i++) {
+    uint32_t ui = u.v[i];
+    u.v[i] = ui + t.v[i];
+    v.v[i] = (ui + C[i]) - t.v[i];
+  }
+}
+
+// Butterfly with twiddle -w: (u, v) <- (u - w v, u + w v).
+__device__ __forceinline__ void fr_butterfly_neg(Fr9 &u, Fr9 &v, const Fr9 &w) {
+  constexpr uint32_t C[9] = ZNTT_FR_C4R;
+  Fr9 t = fr_mul(v, w);
+  fr_norm(u);
+#pragma unroll
+  for (int i = 0; i < 9; i++) {
+    uint32_t ui = u.v[i];
+    v.v[i] = ui + t.v[i];
+    u.v[i] = (ui + C[i]) - t.v[i];
+  }
+}
+
+// x <- x - r if x >= r (x carry-normalised, limbs < 2^29, value < 2^261).
+__device__ __forceinline__ void fr_csub_r(Fr9 &x) {
+  constexpr uint32_t P[9] = ZNTT_FR_P29;
+  Fr9 d;
+  uint32_t borrow = 0;
+#pragma unroll
+  for (int i = 0; i < 9; i++) {
+    uint32_t s = x.v[i] - P[i] - borrow;
+    borrow = s >> 31;
+    d.v[i] = s & kM29;
+  }
+  // top limb: s = x8 - P8 - b is exact in 32 bits; no borrow => x >= r
+  const bool ge = borrow == 0;
+#pragma unroll
+  for (int i = 0; i < 9; i++) x.v[i] = ge ? d.v[i] : x.v[i];
+}
+
+// Full reduction to the canonical representative in [0, r).
+// Requires value < 2^261 (guaranteed by the lazy-reduction bound).
+__device__ __forceinline__ void fr_canon(Fr9 &x) {
+  constexpr uint32_t P[9] = ZNTT_FR_P29;
+  fr_norm(x);
+  // q <= floor(x / r) and floor(x / r) - q <= 2 (see file comment).
+  const uint32_t q = (uint32_t)(((uint64_t)x.v[8] * kFrQMagic) >> 40);
+  int64_t acc = 0;
+#pragma unroll
+  for (int i = 0; i < 9; i++) {
+    acc += (int64_t)x.v[i] - (int64_t)((uint64_t)q * P[i]);
+    x.v[i] = (uint32_t)acc & kM29;
+    acc >>= 29;  // arithmetic shift: signed carry
+  }
+  x.v[8] = (uint32_t)(x.v[8] | ((uint32_t)acc << 29));  // acc == 0 here: x - q r >= 0
+  fr_csub_r(x);
+  fr_csub_r(x);
+}
+
+// 8 little-endian u32 words (value < 2^256) -> nine 29-bit limbs.
+__device__ __forceinline__ Fr9 fr_unpack8(const uint32_t w[8]) {
+  Fr9 x;
+  x.v[0] = w[0] & kM29;
+  x.v[1] = __builtin_amdgcn_alignbit(w[1], w[0], 29) & kM29;
+  x.v[2] = __builtin_amdgcn_alignbit(w[2], w[1], 26) & kM29;
+  x.v[3] = __builtin_amdgcn_alignbit(w[3], w[2], 23) & kM29;
+  x.v[4] = __builtin_amdgcn_alignbit(w[4], w[3], 20) & kM29;
+  x.v[5] = __builtin_amdgcn_alignbit(w[5], w[4], 17) & kM29;
+  x.v[6] = __builtin_amdgcn_alignbit(w[6], w[5], 14) & kM29;
+  x.v[7] = __builtin_amdgcn_alignbit(w[7], w[6], 11) & kM29;
+  x.v[8] = w[7] >> 8;
+  return x;
+}
+
+// Canonical carry-normalised limbs (value < 2^256) -> 8 u32 words.
+__device__ __forceinline__ void fr_pack8(const Fr9 &x, uint32_t w[8]) {
+  w[0] = x.v[0] | (x.v[1] << 29);
+  w[1] = (x.v[1] >> 3) | (x.v[2] << 26);
+  w[2] = (x.v[2] >> 6) | (x.v[3] << 23);
+  w[3] = (x.v[3] >> 9) | (x.v[4] << 20);
+  w[4] = (x.v[4] >> 12) | (x.v[5] << 17);
+  w[5] = (x.v[5] >> 15) | (x.v[6] << 14);
+  w[6] = (x.v[6] >> 18) | (x.v[7] << 11);
+  w[7] = (x.v[7] >> 21) | (x.v[8] << 8);
+}
+
+// Canonical negation r - x for a canonical x (0 stays 0).
+__device__ __forceinline__ Fr9 fr_neg_canon(const Fr9 &x) {
+  constexpr uint32_t P[9] = ZNTT_FR_P29;
+  Fr9 d;
+  uint32_t borrow = 0, nz = 0;
+#pragma unroll
+  for (int i = 0; i < 9; i++) {
+    uint32_t s = P[i] - x.v[i] - borrow;
+    borrow = s >> 31;
+    d.v[i] = s & kM29;
+    nz |= x.v[i];
+  }
+#pragma unroll
+  for (int i = 0; i < 9; i++) d.v[i] = nz ? d.v[i] : 0u;
+  return d;
+}
+
+}  // namespace zntt

@@ -1,0 +1,67 @@
+// Host-side launch interface of the HIP kernels (ntt_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace zntt {
+
+enum FieldId { kFr = 1, kGl = 2 };
+
+// Words (u32) per element in the kernels' two layouts.
+//  user: the reference's storage layout (Fr 8, Goldilocks 2)
+//  dev:  scratch buffers and twiddle tables (Fr 9 radix-2^29 limbs, Goldilocks 2)
+inline int user_words(int field) { return field == kFr ? 8 : 2; }
+inline int dev_words(int field) { return field == kFr ? 9 : 2; }
+
+// One pass covers DIT stages s0 .. s0+K-1 (1-based) of a 2^L-point transform.
+struct PassArgs {
+  const uint32_t *src;     // first pass: user layout, else dev layout (AoS)
+  uint32_t *dst;           // last pass: user layout, else dev layout (AoS)
+  const uint32_t *tw;      // flat per-stage table, dev layout; stage s at 2^(s-1)-1
+  const uint32_t *coset;   // first pass only: shift^i for i < 2^L (dev), or null
+  uint32_t scale[9];       // last pass only: dev-form multiplier when has_scale
+  int has_scale;
+  int L, s0, K, log_g;     // tile = 2^log_g groups of 2^K elements
+  int first, last;
+};
+
+struct PassPlan {
+  int passes;
+  int K[8];
+  int log_g[8];
+  int tile_log;  // log2 of elements per workgroup tile
+};
+
+// Stage split of a 2^L-point transform into LDS passes.
+PassPlan plan_passes(int field, int L);
+
+// Enqueue a full transform over `data` (user layout, 2^L elements, in place).
+// `tw` is a flat dev-layout table holding at least 2^L - 1 entries.
+// `scratch` must hold 2^L dev-layout elements when L exceeds one tile.
+hipError_t launch_transform(int field, int L, uint32_t *data, uint32_t *scratch, const uint32_t *tw,
+                            const uint32_t *coset, const uint32_t *scale_dev, hipStream_t stream);
+
+// Twiddle table builders (dev layout).  `flat` holds 2^L entries: stages
+// 1..L at offsets 2^(s-1)-1 and a trailing one (index 2^L - 1).
+//   pows: dev-form root^(2^k), k < L (device buffer, L entries)
+hipError_t launch_build_table(int field, int L, uint32_t *flat, const uint32_t *pows, const uint32_t *one_dev,
+                              hipStream_t stream);
+// Powers base^i for i < count (dev layout), pows = base^(2^k) for k < 64.
+hipError_t launch_powers(int field, size_t count, uint32_t *out, const uint32_t *pows, int npows,
+                         const uint32_t *one_dev, hipStream_t stream);
+// Dev-layout table from user-layout entries (count entries).
+hipError_t launch_table_from_user(int field, size_t count, uint32_t *dev, const uint32_t *user, hipStream_t stream);
+// User-layout entries from a dev-layout table (canonical).
+hipError_t launch_table_to_user(int field, size_t count, uint32_t *user, const uint32_t *dev, hipStream_t stream);
+// Inverse table for inttWithTwiddles: inv[s][j] = j == 0 ? 1 : -fwd[s][h - j].
+hipError_t launch_inverse_table(int field, int L, uint32_t *inv, const uint32_t *fwd, hipStream_t stream);
+// In-place bit-reversal permutation of n = 2^L user-layout elements.
+hipError_t launch_bit_reverse(int field, int L, uint32_t *data, hipStream_t stream);
+// Elementwise device field op (0 add, 1 sub, 2 mul) on user-layout arrays.
+hipError_t launch_field_op(int field, int op, size_t n, const uint32_t *a, const uint32_t *b, uint32_t *out,
+                           hipStream_t stream);
+// Microbenchmarks: v_mad_u64_u32 rate (mul/s) and device copy bandwidth (B/s).
+hipError_t probe_peaks(double *mad_per_s, double *copy_bytes_per_s, hipStream_t stream);
+
+}  // namespace zntt

@@ -1,0 +1,814 @@
+// C ABI of the MI355X NTT library: context, device memory under an HBM budget,
+// the device-resident twiddle cache, and the zig-ntt entry points
+// (libs/ntt/src/root.zig:33-212) with the reference's validation order.
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <algorithm>
+#include <array>
+#include <list>
+#include <memory>
+#include <mutex>
+#include <vector>
+
+#include "host_field.hpp"
+#include "kernels.hpp"
+#include "zntt_gpu.h"
+
+using zntt::host::U256;
+
+namespace {
+
+// One element in the caller's storage layout (Fr 8 words, Goldilocks 2).
+struct Elem {
+  uint32_t w[8];
+  bool operator==(const Elem &o) const { return memcmp(w, o.w, sizeof w) == 0; }
+};
+
+bool field_ok(int field) { return field == ZNTT_FIELD_BN254_FR || field == ZNTT_FIELD_GOLDILOCKS; }
+size_t user_bytes(int field) { return field == ZNTT_FIELD_BN254_FR ? 32 : 8; }
+size_t dev_bytes(int field) { return 4 * (size_t)zntt::dev_words(field); }
+
+Elem load_elem(int field, const void *p) {
+  Elem e = {};
+  memcpy(e.w, p, user_bytes(field));
+  return e;
+}
+
+// --- host field dispatch on storage-layout elements -------------------------
+U256 to_u256(const Elem &e) {
+  U256 x;
+  memcpy(x.l, e.w, 32);
+  return x;
+}
+Elem from_u256(const U256 &x) {
+  Elem e = {};
+  memcpy(e.w, x.l, 32);
+  return e;
+}
+uint64_t to_u64(const Elem &e) { return (uint64_t)e.w[0] | ((uint64_t)e.w[1] << 32); }
+Elem from_u64e(uint64_t v) {
+  Elem e = {};
+  e.w[0] = (uint32_t)v;
+  e.w[1] = (uint32_t)(v >> 32);
+  return e;
+}
+
+Elem f_mul(int field, const Elem &a, const Elem &b) {
+  if (field == ZNTT_FIELD_BN254_FR) return from_u256(zntt::host::Fr::mont(to_u256(a), to_u256(b)));
+  return from_u64e(zntt::host::Gl::mul(to_u64(a), to_u64(b)));
+}
+Elem f_inv(int field, const Elem &a) {
+  if (field == ZNTT_FIELD_BN254_FR) return from_u256(zntt::host::Fr::inv(to_u256(a)));
+  return from_u64e(zntt::host::Gl::inv(to_u64(a)));
+}
+Elem f_from_u64(int field, uint64_t v) {
+  if (field == ZNTT_FIELD_BN254_FR) return from_u256(zntt::host::Fr::from_u64(v));
+  return from_u64e(v % zntt::host::Gl::P);
+}
+Elem f_one(int field) { return f_from_u64(field, 1); }
+Elem f_primitive_root(int field, int log_size) {
+  if (field == ZNTT_FIELD_BN254_FR) return from_u256(zntt::host::Fr::primitive_root(log_size));
+  return from_u64e(zntt::host::Gl::primitive_root(log_size));
+}
+int two_adicity(int field) { return field == ZNTT_FIELD_BN254_FR ? 28 : 32; }
+// Device constant form (Fr: 29-bit limbs of c * 2^261; Goldilocks: the value).
+std::array<uint32_t, 9> f_to_dev(int field, const Elem &a) {
+  std::array<uint32_t, 9> d = {};
+  if (field == ZNTT_FIELD_BN254_FR) {
+    zntt::host::Fr::to_dev(to_u256(a), d.data());
+  } else {
+    d[0] = a.w[0];
+    d[1] = a.w[1];
+  }
+  return d;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Context
+// ---------------------------------------------------------------------------
+struct TwiddleEntry {
+  int field = 0;
+  int L = 0;
+  std::vector<Elem> chain;           // chain[k] = root^(2^k), k = 0..L (powers tables)
+  uint32_t *table = nullptr;         // dev layout, 2^L entries (stages + trailing one)
+  size_t bytes = 0;
+  uint32_t *inv_table = nullptr;     // inverse-with-twiddles table, built on demand
+  size_t inv_bytes = 0;
+  std::vector<const void *> key;     // host stage pointers of a registered table
+  uint64_t last_use = 0;
+};
+
+struct CosetEntry {
+  int field = 0;
+  Elem shift;
+  size_t count = 0;
+  uint32_t *table = nullptr;  // dev layout, shift^i for i < count
+  size_t bytes = 0;
+  uint64_t last_use = 0;
+};
+
+struct zntt_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  uint64_t budget = 0;
+  uint64_t in_use = 0;
+  uint64_t tick = 0;
+  uint64_t protect_from = 0;  // entries used since this tick belong to the running call
+  std::mutex mu;
+  std::list<TwiddleEntry> tables;
+  std::list<CosetEntry> cosets;
+  void *scratch = nullptr;
+  size_t scratch_bytes = 0;
+  void *staging = nullptr;
+  size_t staging_bytes = 0;
+  void *aux = nullptr;  // foreign tables and other per-call temporaries
+  size_t aux_bytes = 0;
+};
+
+namespace {
+
+// Holds the context lock for one API call and shields the cache entries that
+// call touches from eviction by its own later allocations.
+struct CallGuard {
+  std::lock_guard<std::mutex> lock;
+  explicit CallGuard(zntt_ctx *c) : lock(c->mu) { c->protect_from = c->tick + 1; }
+};
+
+int hip_status(hipError_t e) {
+  if (e == hipSuccess) return ZNTT_OK;
+  if (e == hipErrorOutOfMemory) return ZNTT_OUT_OF_MEMORY;
+  return ZNTT_DEVICE_ERROR;
+}
+
+void dev_free(zntt_ctx *c, void *p, size_t bytes) {
+  if (!p) return;
+  hipFree(p);
+  c->in_use -= bytes;
+}
+
+// Frees the least recently used cache entry; false if nothing is evictable.
+bool evict_one(zntt_ctx *c) {
+  auto tb = c->tables.end();
+  for (auto it = c->tables.begin(); it != c->tables.end(); ++it)
+    if (it->key.empty() && it->last_use < c->protect_from &&
+        (tb == c->tables.end() || it->last_use < tb->last_use))
+      tb = it;
+  auto cb = c->cosets.end();
+  for (auto it = c->cosets.begin(); it != c->cosets.end(); ++it)
+    if (it->last_use < c->protect_from && (cb == c->cosets.end() || it->last_use < cb->last_use)) cb = it;
+  const bool have_t = tb != c->tables.end(), have_c = cb != c->cosets.end();
+  if (!have_t && !have_c) return false;
+  hipDeviceSynchronize();  // in-flight work may still read the entry
+  if (have_t && (!have_c || tb->last_use <= cb->last_use)) {
+    dev_free(c, tb->table, tb->bytes);
+    dev_free(c, tb->inv_table, tb->inv_bytes);
+    c->tables.erase(tb);
+  } else {
+    dev_free(c, cb->table, cb->bytes);
+    c->cosets.erase(cb);
+  }
+  return true;
+}
+
+int dev_alloc(zntt_ctx *c, size_t bytes, void **out) {
+  *out = nullptr;
+  if (bytes == 0) return ZNTT_OK;
+  while (c->in_use + bytes > c->budget)
+    if (!evict_one(c)) return ZNTT_OUT_OF_MEMORY;
+  for (;;) {
+    hipError_t e = hipMalloc(out, bytes);
+    if (e == hipSuccess) break;
+    hipGetLastError();
+    if (e != hipErrorOutOfMemory || !evict_one(c)) return hip_status(e);
+  }
+  c->in_use += bytes;
+  return ZNTT_OK;
+}
+
+// Grows a reusable buffer to at least `bytes`.
+int ensure(zntt_ctx *c, void **buf, size_t *have, size_t bytes) {
+  if (*have >= bytes) return ZNTT_OK;
+  if (*buf) {
+    hipStreamSynchronize(c->stream);
+    hipDeviceSynchronize();
+    dev_free(c, *buf, *have);
+    *buf = nullptr;
+    *have = 0;
+  }
+  int st = dev_alloc(c, bytes, buf);
+  if (st == ZNTT_OK) *have = bytes;
+  return st;
+}
+
+std::vector<Elem> root_chain(int field, const Elem &root, int L) {
+  std::vector<Elem> ch(L + 1);
+  ch[0] = root;
+  for (int k = 1; k <= L; k++) ch[k] = f_mul(field, ch[k - 1], ch[k - 1]);
+  return ch;
+}
+
+// Uploads `words` u32 to a fresh device buffer and waits for the copy.
+int upload_small(zntt_ctx *c, const std::vector<uint32_t> &words, hipStream_t s, void **out) {
+  int st = dev_alloc(c, words.size() * 4, out);
+  if (st) return st;
+  hipError_t e = hipMemcpyAsync(*out, words.data(), words.size() * 4, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  return hip_status(e);
+}
+
+// Device table for the DIT DAG of ntt(root) at size 2^L.  A table built for
+// (root', L') with L' >= L and root'^(2^(L'-L)) == root serves the query: stage
+// s of both tables is {root'^(j 2^(L'-s))}, so the smaller table is a prefix.
+int power_table(zntt_ctx *c, int field, const Elem &root, int L, hipStream_t s, TwiddleEntry **out) {
+  for (auto &e : c->tables) {
+    if (e.field != field || !e.key.empty() || e.L < L) continue;
+    if (e.chain[e.L - L] == root) {
+      e.last_use = ++c->tick;
+      *out = &e;
+      return ZNTT_OK;
+    }
+  }
+  TwiddleEntry e;
+  e.field = field;
+  e.L = L;
+  e.chain = root_chain(field, root, L);
+  e.bytes = ((size_t)1 << L) * dev_bytes(field);
+  int st = dev_alloc(c, e.bytes, (void **)&e.table);
+  if (st) return st;
+  std::vector<uint32_t> consts;
+  const int W = zntt::dev_words(field);
+  for (int k = 0; k < std::max(L, 1); k++) {
+    auto d = f_to_dev(field, e.chain[k]);
+    consts.insert(consts.end(), d.begin(), d.begin() + W);
+  }
+  auto one = f_to_dev(field, f_one(field));
+  consts.insert(consts.end(), one.begin(), one.begin() + W);
+  void *cbuf = nullptr;
+  st = upload_small(c, consts, s, &cbuf);
+  if (st) {
+    dev_free(c, e.table, e.bytes);
+    return st;
+  }
+  const uint32_t *pows = (const uint32_t *)cbuf;
+  const uint32_t *one_dev = pows + (size_t)std::max(L, 1) * W;
+  hipError_t he = zntt::launch_build_table(field, L, e.table, pows, one_dev, s);
+  if (he == hipSuccess) he = hipStreamSynchronize(s);
+  dev_free(c, cbuf, consts.size() * 4);
+  if (he != hipSuccess) {
+    dev_free(c, e.table, e.bytes);
+    return hip_status(he);
+  }
+  e.last_use = ++c->tick;
+  c->tables.push_back(std::move(e));
+  *out = &c->tables.back();
+  return ZNTT_OK;
+}
+
+// shift^i for i < count (dev layout), cached with prefix reuse.
+int coset_table(zntt_ctx *c, int field, const Elem &shift, size_t count, hipStream_t s, const uint32_t **out) {
+  for (auto &e : c->cosets) {
+    if (e.field == field && e.count >= count && e.shift == shift) {
+      e.last_use = ++c->tick;
+      *out = e.table;
+      return ZNTT_OK;
+    }
+  }
+  CosetEntry e;
+  e.field = field;
+  e.shift = shift;
+  e.count = count;
+  e.bytes = count * dev_bytes(field);
+  int st = dev_alloc(c, e.bytes, (void **)&e.table);
+  if (st) return st;
+  const int W = zntt::dev_words(field);
+  std::vector<uint32_t> consts;
+  Elem p = shift;
+  int npows = 0;
+  for (; npows < 64 && ((size_t)1 << npows) <= count; npows++) {
+    auto d = f_to_dev(field, p);
+    consts.insert(consts.end(), d.begin(), d.begin() + W);
+    p = f_mul(field, p, p);
+  }
+  if (npows == 0) {
+    auto d = f_to_dev(field, shift);
+    consts.insert(consts.end(), d.begin(), d.begin() + W);
+    npows = 1;
+  }
+  auto one = f_to_dev(field, f_one(field));
+  consts.insert(consts.end(), one.begin(), one.begin() + W);
+  void *cbuf = nullptr;
+  st = upload_small(c, consts, s, &cbuf);
+  if (st) {
+    dev_free(c, e.table, e.bytes);
+    return st;
+  }
+  const uint32_t *pows = (const uint32_t *)cbuf;
+  hipError_t he = zntt::launch_powers(field, count, e.table, pows, npows, pows + (size_t)npows * W, s);
+  if (he == hipSuccess) he = hipStreamSynchronize(s);
+  dev_free(c, cbuf, consts.size() * 4);
+  if (he != hipSuccess) {
+    dev_free(c, e.table, e.bytes);
+    return hip_status(he);
+  }
+  e.last_use = ++c->tick;
+  c->cosets.push_back(std::move(e));
+  *out = c->cosets.back().table;
+  return ZNTT_OK;
+}
+
+int check_transform_args(zntt_ctx *c, int field, const void *data, size_t len, size_t log_n) {
+  if (!c || !field_ok(field)) return ZNTT_INVALID_ARGUMENT;
+  if (log_n >= 64) return ZNTT_LOG_TOO_LARGE;         // root.zig:33-36
+  if (len != ((size_t)1 << log_n)) return ZNTT_LENGTH_MISMATCH;  // root.zig:68
+  if (!data) return ZNTT_INVALID_ARGUMENT;
+  if (log_n > ZNTT_MAX_LOG_N) return ZNTT_UNSUPPORTED;
+  return ZNTT_OK;
+}
+
+// Runs one transform over device data on stream s.  `tw` covers >= 2^L - 1
+// entries; `coset` (nullable) covers 2^L; `scale` (nullable) is a dev constant.
+int run_device(zntt_ctx *c, int field, int L, void *d_data, const uint32_t *tw, const uint32_t *coset,
+               const std::array<uint32_t, 9> *scale, hipStream_t s) {
+  if (L == 0) {
+    if (!scale && !coset) return ZNTT_OK;
+  }
+  const zntt::PassPlan plan = zntt::plan_passes(field, L);
+  if (plan.passes > 1) {
+    int st = ensure(c, &c->scratch, &c->scratch_bytes, ((size_t)1 << L) * dev_bytes(field));
+    if (st) return st;
+  }
+  hipError_t e = zntt::launch_transform(field, L, (uint32_t *)d_data, (uint32_t *)c->scratch, tw, coset,
+                                        scale ? scale->data() : nullptr, s);
+  return hip_status(e);
+}
+
+// Host-buffer wrapper: stage data, run, copy back, wait.
+template <class Fn>
+int with_staging(zntt_ctx *c, int field, void *data, size_t n, Fn &&fn) {
+  const size_t bytes = n * user_bytes(field);
+  int st = ensure(c, &c->staging, &c->staging_bytes, bytes);
+  if (st) return st;
+  hipError_t e = hipMemcpyAsync(c->staging, data, bytes, hipMemcpyHostToDevice, c->stream);
+  if (e != hipSuccess) return hip_status(e);
+  st = fn(c->staging);
+  if (st) {
+    hipStreamSynchronize(c->stream);
+    return st;
+  }
+  e = hipMemcpyAsync(data, c->staging, bytes, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  return hip_status(e);
+}
+
+TwiddleEntry *find_registered(zntt_ctx *c, int field, size_t log_n, const void *const *stages) {
+  for (auto &e : c->tables) {
+    if (e.field != field || e.key.empty() || (size_t)e.L != log_n) continue;
+    bool same = true;
+    for (size_t s = 0; s < log_n && same; s++) same = e.key[s] == stages[s];
+    if (same) {
+      e.last_use = ++c->tick;
+      return &e;
+    }
+  }
+  return nullptr;
+}
+
+// Uploads a caller's table (stage s holds 2^s user-layout elements) into the
+// dev layout at c->aux, followed by the trailing one.
+int upload_foreign(zntt_ctx *c, int field, size_t log_n, const void *const *stages, uint32_t **out) {
+  const size_t n = (size_t)1 << log_n;
+  const size_t ub = user_bytes(field), db = dev_bytes(field);
+  // aux holds: dev table (n entries) + user staging (n entries) + inverse table (n entries)
+  int st = ensure(c, &c->aux, &c->aux_bytes, n * (2 * db + ub));
+  if (st) return st;
+  uint32_t *dev = (uint32_t *)c->aux;
+  uint8_t *user = (uint8_t *)c->aux + n * db;
+  std::vector<uint8_t> host(n * ub);
+  for (size_t s = 0; s < log_n; s++) memcpy(host.data() + (((size_t)1 << s) - 1) * ub, stages[s], ((size_t)1 << s) * ub);
+  const Elem one = f_one(field);
+  memcpy(host.data() + (n - 1) * ub, one.w, ub);
+  hipError_t e = hipMemcpyAsync(user, host.data(), n * ub, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = zntt::launch_table_from_user(field, n, dev, (const uint32_t *)user, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  *out = dev;
+  return hip_status(e);
+}
+
+int twiddled(zntt_ctx *c, int field, void *data, size_t len, size_t log_n, const void *const *stages,
+             const size_t *stage_lens, size_t num_stages, bool inverse) {
+  int st = check_transform_args(c, field, data, len, log_n);
+  if (st) return st;
+  if (num_stages != log_n) return ZNTT_INVALID_TWIDDLES;  // root.zig:152
+  if (log_n > 0 && (!stages || !stage_lens)) return ZNTT_INVALID_ARGUMENT;
+  for (size_t s = 0; s < log_n; s++)  // root.zig:161, checked before data is touched
+    if (stage_lens[s] != ((size_t)1 << s)) return ZNTT_INVALID_TWIDDLES;
+  for (size_t s = 0; s < log_n; s++)
+    if (!stages[s]) return ZNTT_INVALID_ARGUMENT;
+  CallGuard g(c);
+  if (log_n == 0) return ZNTT_OK;
+  const int L = (int)log_n;
+  const uint32_t *tw = nullptr;
+  TwiddleEntry *reg = find_registered(c, field, log_n, stages);
+  if (reg) {
+    tw = reg->table;
+    if (inverse) {
+      if (!reg->inv_table) {
+        reg->inv_bytes = reg->bytes;
+        st = dev_alloc(c, reg->inv_bytes, (void **)&reg->inv_table);
+        if (st) return st;
+        hipError_t e = zntt::launch_inverse_table(field, L, reg->inv_table, reg->table, c->stream);
+        if (e != hipSuccess) return hip_status(e);
+      }
+      tw = reg->inv_table;
+    }
+  } else {
+    uint32_t *dev = nullptr;
+    st = upload_foreign(c, field, log_n, stages, &dev);
+    if (st) return st;
+    tw = dev;
+    if (inverse) {
+      const size_t n = (size_t)1 << log_n;
+      uint32_t *inv = (uint32_t *)((uint8_t *)c->aux + n * (dev_bytes(field) + user_bytes(field)));
+      hipError_t e = zntt::launch_inverse_table(field, L, inv, dev, c->stream);
+      if (e != hipSuccess) return hip_status(e);
+      tw = inv;
+    }
+  }
+  std::array<uint32_t, 9> scale;
+  if (inverse) scale = f_to_dev(field, f_inv(field, f_from_u64(field, len)));  // root.zig:208-211
+  return with_staging(c, field, data, len, [&](void *d) {
+    return run_device(c, field, L, d, tw, nullptr, inverse ? &scale : nullptr, c->stream);
+  });
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Exported entry points
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int zntt_ctx_create(zntt_ctx **out, int device, uint64_t hbm_budget_bytes) {
+  if (!out) return ZNTT_INVALID_ARGUMENT;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+    hipGetLastError();
+    return ZNTT_NO_DEVICE;
+  }
+  if (device < 0 || device >= count) return ZNTT_NO_DEVICE;
+  if (hipSetDevice(device) != hipSuccess) return ZNTT_DEVICE_ERROR;
+  auto *c = new zntt_ctx();
+  c->device = device;
+  c->budget = hbm_budget_bytes ? hbm_budget_bytes : (64ull << 30);
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return ZNTT_DEVICE_ERROR;
+  }
+  *out = c;
+  return ZNTT_OK;
+}
+
+int zntt_ctx_trim(zntt_ctx *c) {
+  if (!c) return ZNTT_INVALID_ARGUMENT;
+  CallGuard g(c);
+  hipStreamSynchronize(c->stream);
+  hipDeviceSynchronize();
+  for (auto &e : c->tables) {
+    dev_free(c, e.table, e.bytes);
+    dev_free(c, e.inv_table, e.inv_bytes);
+  }
+  c->tables.clear();
+  for (auto &e : c->cosets) dev_free(c, e.table, e.bytes);
+  c->cosets.clear();
+  dev_free(c, c->scratch, c->scratch_bytes);
+  dev_free(c, c->staging, c->staging_bytes);
+  dev_free(c, c->aux, c->aux_bytes);
+  c->scratch = c->staging = c->aux = nullptr;
+  c->scratch_bytes = c->staging_bytes = c->aux_bytes = 0;
+  return ZNTT_OK;
+}
+
+void zntt_ctx_destroy(zntt_ctx *c) {
+  if (!c) return;
+  zntt_ctx_trim(c);
+  hipStreamDestroy(c->stream);
+  delete c;
+}
+
+uint64_t zntt_ctx_bytes_in_use(const zntt_ctx *c) { return c ? c->in_use : 0; }
+
+int zntt_ctx_synchronize(zntt_ctx *c) {
+  if (!c) return ZNTT_INVALID_ARGUMENT;
+  return hip_status(hipStreamSynchronize(c->stream));
+}
+
+const char *zntt_status_string(int status) {
+  switch (status) {
+    case ZNTT_OK: return "ok";
+    case ZNTT_LOG_TOO_LARGE: return "LogTooLarge";
+    case ZNTT_LENGTH_MISMATCH: return "LengthMismatch";
+    case ZNTT_INVALID_LENGTH: return "InvalidLength";
+    case ZNTT_INVALID_TWIDDLES: return "InvalidTwiddles";
+    case ZNTT_OUT_OF_MEMORY: return "OutOfMemory";
+    case ZNTT_DEVICE_ERROR: return "DeviceError";
+    case ZNTT_NO_DEVICE: return "NoDevice";
+    case ZNTT_INVALID_ARGUMENT: return "InvalidArgument";
+    case ZNTT_UNSUPPORTED: return "Unsupported";
+    case ZNTT_ORDER_TOO_LARGE: return "OrderTooLarge";
+    default: return "unknown";
+  }
+}
+
+size_t zntt_element_size(int field) { return field_ok(field) ? user_bytes(field) : 0; }
+
+int zntt_primitive_root(int field, size_t log_size, void *out) {
+  if (!field_ok(field) || !out) return ZNTT_INVALID_ARGUMENT;
+  if (log_size > (size_t)two_adicity(field)) return ZNTT_ORDER_TOO_LARGE;
+  const Elem r = f_primitive_root(field, (int)log_size);
+  memcpy(out, r.w, user_bytes(field));
+  return ZNTT_OK;
+}
+
+int zntt_bit_reverse(zntt_ctx *c, int field, void *data, size_t len) {
+  if (!c || !field_ok(field)) return ZNTT_INVALID_ARGUMENT;
+  if (len == 0 || (len & (len - 1)) != 0) return ZNTT_INVALID_LENGTH;  // root.zig:43
+  if (!data) return ZNTT_INVALID_ARGUMENT;
+  const int L = __builtin_ctzll(len);
+  if (L > ZNTT_MAX_LOG_N) return ZNTT_UNSUPPORTED;
+  CallGuard g(c);
+  return with_staging(c, field, data, len, [&](void *d) {
+    return hip_status(zntt::launch_bit_reverse(field, L, (uint32_t *)d, c->stream));
+  });
+}
+
+int zntt_ntt(zntt_ctx *c, int field, void *data, size_t len, size_t log_n, const void *root) {
+  int st = check_transform_args(c, field, data, len, log_n);
+  if (st) return st;
+  if (!root) return ZNTT_INVALID_ARGUMENT;
+  CallGuard g(c);
+  if (log_n == 0) return ZNTT_OK;
+  TwiddleEntry *t = nullptr;
+  st = power_table(c, field, load_elem(field, root), (int)log_n, c->stream, &t);
+  if (st) return st;
+  return with_staging(c, field, data, len, [&](void *d) {
+    return run_device(c, field, (int)log_n, d, t->table, nullptr, nullptr, c->stream);
+  });
+}
+
+int zntt_intt(zntt_ctx *c, int field, void *data, size_t len, size_t log_n, const void *root) {
+  int st = check_transform_args(c, field, data, len, log_n);
+  if (st) return st;
+  if (!root) return ZNTT_INVALID_ARGUMENT;
+  CallGuard g(c);
+  if (log_n == 0) return ZNTT_OK;  // n^-1 = 1
+  const Elem root_inv = f_inv(field, load_elem(field, root));  // root.zig:100
+  TwiddleEntry *t = nullptr;
+  st = power_table(c, field, root_inv, (int)log_n, c->stream, &t);
+  if (st) return st;
+  const std::array<uint32_t, 9> scale = f_to_dev(field, f_inv(field, f_from_u64(field, len)));  // root.zig:103
+  return with_staging(c, field, data, len, [&](void *d) {
+    return run_device(c, field, (int)log_n, d, t->table, nullptr, &scale, c->stream);
+  });
+}
+
+int zntt_precompute_twiddles(zntt_ctx *c, int field, size_t log_n, const void *root, void *const *stages,
+                             size_t num_stages) {
+  if (!c || !field_ok(field)) return ZNTT_INVALID_ARGUMENT;
+  if (log_n >= 64) return ZNTT_LOG_TOO_LARGE;  // root.zig:114
+  if (num_stages != log_n || !root || (log_n > 0 && !stages)) return ZNTT_INVALID_ARGUMENT;
+  for (size_t s = 0; s < log_n; s++)
+    if (!stages[s]) return ZNTT_INVALID_ARGUMENT;
+  if (log_n > ZNTT_MAX_LOG_N) return ZNTT_OUT_OF_MEMORY;
+  CallGuard g(c);
+  if (log_n == 0) return ZNTT_OK;
+  const int L = (int)log_n;
+  // Drop a stale registration of the same host buffers.
+  if (TwiddleEntry *old = find_registered(c, field, log_n, (const void *const *)stages)) {
+    dev_free(c, old->table, old->bytes);
+    dev_free(c, old->inv_table, old->inv_bytes);
+    c->tables.remove_if([&](const TwiddleEntry &e) { return &e == old; });
+  }
+  TwiddleEntry *src = nullptr;
+  int st = power_table(c, field, load_elem(field, root), L, c->stream, &src);
+  if (st) return st;
+  // A registered entry owns its own copy of the table (the cache may evict
+  // power tables; registered ones live until zntt_free_twiddles).
+  TwiddleEntry e;
+  e.field = field;
+  e.L = L;
+  e.bytes = ((size_t)1 << L) * dev_bytes(field);
+  st = dev_alloc(c, e.bytes, (void **)&e.table);
+  if (st) return st;
+  const size_t ub = user_bytes(field);
+  const size_t n = (size_t)1 << L;
+  st = ensure(c, &c->staging, &c->staging_bytes, n * ub);
+  if (st) {
+    dev_free(c, e.table, e.bytes);
+    return st;
+  }
+  // src may cover a larger size: copy the prefix of 2^L - 1 entries, then one.
+  hipError_t he = hipMemcpyAsync(e.table, src->table, (n - 1) * dev_bytes(field), hipMemcpyDeviceToDevice, c->stream);
+  if (he == hipSuccess)
+    he = hipMemcpyAsync(e.table + (n - 1) * zntt::dev_words(field),
+                        src->table + (((size_t)1 << src->L) - 1) * zntt::dev_words(field), dev_bytes(field),
+                        hipMemcpyDeviceToDevice, c->stream);
+  if (he == hipSuccess) he = zntt::launch_table_to_user(field, n - 1, (uint32_t *)c->staging, e.table, c->stream);
+  std::vector<uint8_t> host((n - 1) * ub);
+  if (he == hipSuccess) he = hipMemcpyAsync(host.data(), c->staging, (n - 1) * ub, hipMemcpyDeviceToHost, c->stream);
+  if (he == hipSuccess) he = hipStreamSynchronize(c->stream);
+  if (he != hipSuccess) {
+    dev_free(c, e.table, e.bytes);
+    return hip_status(he);
+  }
+  for (size_t s = 0; s < log_n; s++) memcpy(stages[s], host.data() + (((size_t)1 << s) - 1) * ub, ((size_t)1 << s) * ub);
+  e.key.assign(stages, stages + log_n);
+  e.last_use = ++c->tick;
+  c->tables.push_back(std::move(e));
+  return ZNTT_OK;
+}
+
+int zntt_free_twiddles(zntt_ctx *c, const void *const *stages, size_t num_stages) {
+  if (!c) return ZNTT_INVALID_ARGUMENT;
+  if (num_stages == 0) return ZNTT_OK;
+  if (!stages) return ZNTT_INVALID_ARGUMENT;
+  CallGuard g(c);
+  for (auto it = c->tables.begin(); it != c->tables.end(); ++it) {
+    if (it->key.size() != num_stages) continue;
+    if (!std::equal(it->key.begin(), it->key.end(), stages)) continue;
+    hipStreamSynchronize(c->stream);
+    dev_free(c, it->table, it->bytes);
+    dev_free(c, it->inv_table, it->inv_bytes);
+    c->tables.erase(it);
+    return ZNTT_OK;
+  }
+  return ZNTT_OK;  // not registered (e.g. built elsewhere): nothing on the device
+}
+
+int zntt_ntt_with_twiddles(zntt_ctx *c, int field, void *data, size_t len, size_t log_n, const void *const *stages,
+                           const size_t *stage_lens, size_t num_stages) {
+  return twiddled(c, field, data, len, log_n, stages, stage_lens, num_stages, false);
+}
+
+int zntt_intt_with_twiddles(zntt_ctx *c, int field, void *data, size_t len, size_t log_n, const void *const *stages,
+                            const size_t *stage_lens, size_t num_stages) {
+  return twiddled(c, field, data, len, log_n, stages, stage_lens, num_stages, true);
+}
+
+int zntt_coset_ntt(zntt_ctx *c, int field, void *data, size_t len, size_t log_n, const void *root,
+                   const void *shift) {
+  int st = check_transform_args(c, field, data, len, log_n);
+  if (st) return st;
+  if (!root || !shift) return ZNTT_INVALID_ARGUMENT;
+  CallGuard g(c);
+  TwiddleEntry *t = nullptr;
+  if (log_n > 0) {
+    st = power_table(c, field, load_elem(field, root), (int)log_n, c->stream, &t);
+    if (st) return st;
+  }
+  const uint32_t *cos = nullptr;
+  st = coset_table(c, field, load_elem(field, shift), len, c->stream, &cos);
+  if (st) return st;
+  if (log_n == 0) return ZNTT_OK;  // data[0] *= shift^0
+  return with_staging(c, field, data, len, [&](void *d) {
+    return run_device(c, field, (int)log_n, d, t->table, cos, nullptr, c->stream);
+  });
+}
+
+int zntt_lde(zntt_ctx *c, int field, const void *coeffs, size_t log_k, size_t log_blowup, const void *shift,
+             void *out) {
+  if (!c || !field_ok(field) || !coeffs || !shift || !out) return ZNTT_INVALID_ARGUMENT;
+  if (log_k >= 64 || log_blowup >= 64 || log_k + log_blowup >= 64) return ZNTT_LOG_TOO_LARGE;
+  const size_t L = log_k + log_blowup;
+  if (L > (size_t)two_adicity(field)) return ZNTT_ORDER_TOO_LARGE;
+  if (L > ZNTT_MAX_LOG_N) return ZNTT_UNSUPPORTED;
+  const size_t k = (size_t)1 << log_k, n = (size_t)1 << L, ub = user_bytes(field);
+  CallGuard g(c);
+  TwiddleEntry *t = nullptr;
+  const Elem root = f_primitive_root(field, (int)L);
+  int st = ZNTT_OK;
+  if (L > 0) {
+    st = power_table(c, field, root, (int)L, c->stream, &t);
+    if (st) return st;
+  }
+  const uint32_t *cos = nullptr;
+  st = coset_table(c, field, load_elem(field, shift), n, c->stream, &cos);
+  if (st) return st;
+  st = ensure(c, &c->staging, &c->staging_bytes, n * ub);
+  if (st) return st;
+  hipError_t e = hipMemsetAsync((uint8_t *)c->staging + k * ub, 0, (n - k) * ub, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(c->staging, coeffs, k * ub, hipMemcpyHostToDevice, c->stream);
+  if (e != hipSuccess) return hip_status(e);
+  if (L == 0) {
+    // f(shift) with a single coefficient is the coefficient itself.
+    memcpy(out, coeffs, ub);
+    return hip_status(hipStreamSynchronize(c->stream));
+  }
+  st = run_device(c, field, (int)L, c->staging, t->table, cos, nullptr, c->stream);
+  if (st) return st;
+  e = hipMemcpyAsync(out, c->staging, n * ub, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  return hip_status(e);
+}
+
+int zntt_execute_device(zntt_ctx *c, const zntt_transform *t, void *stream) {
+  if (!c || !t || !field_ok(t->field) || !t->root) return ZNTT_INVALID_ARGUMENT;
+  if (t->log_n >= 64) return ZNTT_LOG_TOO_LARGE;
+  if (t->log_n > ZNTT_MAX_LOG_N) return ZNTT_UNSUPPORTED;
+  if (!t->data) return ZNTT_INVALID_ARGUMENT;
+  if (t->direction == ZNTT_COSET_FORWARD && !t->shift) return ZNTT_INVALID_ARGUMENT;
+  if (t->direction < ZNTT_FORWARD || t->direction > ZNTT_COSET_FORWARD) return ZNTT_INVALID_ARGUMENT;
+  hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+  CallGuard g(c);
+  const int field = t->field, L = (int)t->log_n;
+  const size_t n = (size_t)1 << L;
+  Elem root = load_elem(field, t->root);
+  std::array<uint32_t, 9> scale;
+  const std::array<uint32_t, 9> *scale_p = nullptr;
+  if (t->direction == ZNTT_INVERSE) {
+    root = f_inv(field, root);
+    scale = f_to_dev(field, f_inv(field, f_from_u64(field, n)));
+    scale_p = &scale;
+  }
+  TwiddleEntry *tw = nullptr;
+  int st = ZNTT_OK;
+  if (L > 0) {
+    st = power_table(c, field, root, L, s, &tw);
+    if (st) return st;
+  }
+  const uint32_t *cos = nullptr;
+  if (t->direction == ZNTT_COSET_FORWARD) {
+    st = coset_table(c, field, load_elem(field, t->shift), n, s, &cos);
+    if (st) return st;
+  }
+  if (L == 0) return ZNTT_OK;
+  return run_device(c, field, L, t->data, tw->table, cos, scale_p, s);
+}
+
+int zntt_debug_field_op(zntt_ctx *c, int field, int op, const void *a, const void *b, void *out, size_t n) {
+  if (!c || !field_ok(field) || !a || !b || !out || op < 0 || op > 2) return ZNTT_INVALID_ARGUMENT;
+  CallGuard g(c);
+  const size_t bytes = n * user_bytes(field);
+  void *buf = nullptr;
+  int st = dev_alloc(c, 3 * bytes, &buf);
+  if (st) return st;
+  uint8_t *da = (uint8_t *)buf, *db = da + bytes, *dout = db + bytes;
+  hipError_t e = hipMemcpyAsync(da, a, bytes, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(db, b, bytes, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess)
+    e = zntt::launch_field_op(field, op, n, (const uint32_t *)da, (const uint32_t *)db, (uint32_t *)dout, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(out, dout, bytes, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  dev_free(c, buf, 3 * bytes);
+  return hip_status(e);
+}
+
+int zntt_fill_uniform(int field, uint64_t seed, void *out, size_t n) {
+  if (!field_ok(field) || (!out && n)) return ZNTT_INVALID_ARGUMENT;
+  // SplitMix64 expands the seed into the xoshiro256++ state.
+  uint64_t sm = seed, st[4];
+  for (int i = 0; i < 4; i++) {
+    uint64_t z = (sm += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    st[i] = z ^ (z >> 31);
+  }
+  auto next = [&st]() {
+    auto rotl = [](uint64_t x, int k) { return (x << k) | (x >> (64 - k)); };
+    const uint64_t r = rotl(st[0] + st[3], 23) + st[0];
+    const uint64_t t = st[1] << 17;
+    st[2] ^= st[0];
+    st[3] ^= st[1];
+    st[1] ^= st[2];
+    st[0] ^= st[3];
+    st[2] ^= t;
+    st[3] = rotl(st[3], 45);
+    return r;
+  };
+  if (field == ZNTT_FIELD_BN254_FR) {
+    U256 *o = (U256 *)out;
+    for (size_t i = 0; i < n;) {
+      U256 v;
+      for (int k = 0; k < 4; k++) v.l[k] = next();
+      if (zntt::host::Fr::lt(v, zntt::host::Fr::P)) o[i++] = zntt::host::Fr::from_canonical(v);
+    }
+  } else {
+    uint64_t *o = (uint64_t *)out;
+    for (size_t i = 0; i < n;) {
+      const uint64_t v = next();
+      if (v < zntt::host::Gl::P) o[i++] = v;
+    }
+  }
+  return ZNTT_OK;
+}
+
+int zntt_probe_peaks(zntt_ctx *c, double *mad_per_s, double *copy_bytes_per_s) {
+  if (!c || !mad_per_s || !copy_bytes_per_s) return ZNTT_INVALID_ARGUMENT;
+  CallGuard g(c);
+  return hip_status(zntt::probe_peaks(mad_per_s, copy_bytes_per_s, c->stream));
+}
+
+}  // extern "C"

@@ -1,0 +1,288 @@
+"""Python host binding of libzntt_gpu.so, the MI355X NTT library.
+
+Mirrors zig-ntt's interface (``libs/ntt/src/root.zig``): the same seven
+functions, the same argument meaning and the same typed errors, raised as
+exceptions named after the Zig error values.  Every call goes through the C
+ABI declared in ``include/zntt_gpu.h``; there is no CPU fallback -- importing
+this module without the built library raises immediately.
+
+Arrays use the reference's element layout:
+  * BN254 Fr: numpy ``uint64`` of shape ``(n, 4)`` (Montgomery limbs, R = 2^256);
+  * Goldilocks: numpy ``uint64`` of shape ``(n,)`` (canonical values).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+FIELD_FR = 1
+FIELD_GL = 2
+
+FORWARD = 0
+INVERSE = 1
+COSET_FORWARD = 2
+
+MAX_LOG_N = 28
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libzntt_gpu.so")
+
+
+class ZnttError(Exception):
+    """Base class; ``status`` is the C ABI status code."""
+    status = -1
+
+
+class LogTooLarge(ZnttError):
+    status = 1
+
+
+class LengthMismatch(ZnttError):
+    status = 2
+
+
+class InvalidLength(ZnttError):
+    status = 3
+
+
+class InvalidTwiddles(ZnttError):
+    status = 4
+
+
+class OutOfMemory(ZnttError):
+    status = 5
+
+
+class DeviceError(ZnttError):
+    status = 6
+
+
+class NoDevice(ZnttError):
+    status = 7
+
+
+class InvalidArgument(ZnttError):
+    status = 8
+
+
+class Unsupported(ZnttError):
+    status = 9
+
+
+class OrderTooLarge(ZnttError):
+    status = 10
+
+
+_ERRORS = {c.status: c for c in (LogTooLarge, LengthMismatch, InvalidLength, InvalidTwiddles, OutOfMemory,
+                                 DeviceError, NoDevice, InvalidArgument, Unsupported, OrderTooLarge)}
+
+# Every symbol include/zntt_gpu.h declares.
+EXPORTED = [
+    "zntt_ctx_create", "zntt_ctx_destroy", "zntt_ctx_bytes_in_use", "zntt_ctx_trim", "zntt_status_string",
+    "zntt_element_size", "zntt_bit_reverse", "zntt_ntt", "zntt_intt", "zntt_precompute_twiddles",
+    "zntt_free_twiddles", "zntt_ntt_with_twiddles", "zntt_intt_with_twiddles", "zntt_coset_ntt", "zntt_lde",
+    "zntt_primitive_root", "zntt_execute_device", "zntt_ctx_synchronize", "zntt_debug_field_op",
+    "zntt_probe_peaks", "zntt_fill_uniform",
+]
+
+
+class Transform(ctypes.Structure):
+    _fields_ = [("field", ctypes.c_int), ("direction", ctypes.c_int), ("log_n", ctypes.c_size_t),
+                ("data", ctypes.c_void_p), ("root", ctypes.c_void_p), ("shift", ctypes.c_void_p)]
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built; run `make -C zig-algebra_amd` (or __graft_entry__.build())")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, sz, i32, u64 = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_uint64
+    pp = ctypes.POINTER(ctypes.c_void_p)
+    L.zntt_ctx_create.argtypes = [ctypes.POINTER(vp), i32, u64]
+    L.zntt_ctx_destroy.argtypes = [vp]
+    L.zntt_ctx_destroy.restype = None
+    L.zntt_ctx_bytes_in_use.argtypes = [vp]
+    L.zntt_ctx_bytes_in_use.restype = u64
+    L.zntt_ctx_trim.argtypes = [vp]
+    L.zntt_status_string.argtypes = [i32]
+    L.zntt_status_string.restype = ctypes.c_char_p
+    L.zntt_element_size.argtypes = [i32]
+    L.zntt_element_size.restype = sz
+    L.zntt_bit_reverse.argtypes = [vp, i32, vp, sz]
+    L.zntt_ntt.argtypes = [vp, i32, vp, sz, sz, vp]
+    L.zntt_intt.argtypes = [vp, i32, vp, sz, sz, vp]
+    L.zntt_precompute_twiddles.argtypes = [vp, i32, sz, vp, pp, sz]
+    L.zntt_free_twiddles.argtypes = [vp, pp, sz]
+    L.zntt_ntt_with_twiddles.argtypes = [vp, i32, vp, sz, sz, pp, ctypes.POINTER(sz), sz]
+    L.zntt_intt_with_twiddles.argtypes = [vp, i32, vp, sz, sz, pp, ctypes.POINTER(sz), sz]
+    L.zntt_coset_ntt.argtypes = [vp, i32, vp, sz, sz, vp, vp]
+    L.zntt_lde.argtypes = [vp, i32, vp, sz, sz, vp, vp]
+    L.zntt_primitive_root.argtypes = [i32, sz, vp]
+    L.zntt_execute_device.argtypes = [vp, ctypes.POINTER(Transform), vp]
+    L.zntt_ctx_synchronize.argtypes = [vp]
+    L.zntt_debug_field_op.argtypes = [vp, i32, i32, vp, vp, vp, sz]
+    L.zntt_probe_peaks.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+    L.zntt_fill_uniform.argtypes = [i32, u64, vp, sz]
+    return L
+
+
+lib = _load()
+
+
+def check(status: int) -> None:
+    if status:
+        cls = _ERRORS.get(status, ZnttError)
+        raise cls(lib.zntt_status_string(status).decode())
+
+
+def element_size(field: int) -> int:
+    return int(lib.zntt_element_size(field))
+
+
+def _ptr(a: np.ndarray | None):
+    if a is None:
+        return None
+    assert isinstance(a, np.ndarray) and a.flags["C_CONTIGUOUS"] and a.dtype == np.uint64
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _elem(field: int, x) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(x, dtype=np.uint64).reshape(-1))
+
+
+def empty(field: int, n: int) -> np.ndarray:
+    return np.zeros((n, 4) if field == FIELD_FR else (n,), dtype=np.uint64)
+
+
+def fill_uniform(field: int, seed: int, n: int) -> np.ndarray:
+    """Seeded uniform elements in storage layout (host-side generator)."""
+    out = empty(field, n)
+    check(lib.zntt_fill_uniform(field, seed, _ptr(out), n))
+    return out
+
+
+def primitive_root(field: int, log_size: int) -> np.ndarray:
+    """Field's primitive 2^log_size-th root (libs/field/src/roots.zig:149-157)."""
+    out = empty(field, 1).reshape(-1)
+    check(lib.zntt_primitive_root(field, log_size, _ptr(out)))
+    return out
+
+
+class Context:
+    """One GPU: a HIP stream, scratch memory and the device twiddle cache."""
+
+    def __init__(self, device: int = 0, hbm_budget_bytes: int = 0):
+        h = ctypes.c_void_p()
+        check(lib.zntt_ctx_create(ctypes.byref(h), device, hbm_budget_bytes))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if self._h:
+            lib.zntt_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def bytes_in_use(self) -> int:
+        return int(lib.zntt_ctx_bytes_in_use(self._h))
+
+    def trim(self) -> None:
+        check(lib.zntt_ctx_trim(self._h))
+
+    def synchronize(self) -> None:
+        check(lib.zntt_ctx_synchronize(self._h))
+
+    # ---- the seven zig-ntt functions (root.zig:38-212) ---------------------
+    def bit_reverse(self, field: int, data: np.ndarray) -> None:
+        check(lib.zntt_bit_reverse(self._h, field, _ptr(data), data.shape[0]))
+
+    def ntt(self, field: int, data: np.ndarray, log_n: int, root) -> None:
+        check(lib.zntt_ntt(self._h, field, _ptr(data), data.shape[0], log_n, _ptr(_elem(field, root))))
+
+    def intt(self, field: int, data: np.ndarray, log_n: int, root) -> None:
+        check(lib.zntt_intt(self._h, field, _ptr(data), data.shape[0], log_n, _ptr(_elem(field, root))))
+
+    def precompute_twiddles(self, field: int, log_n: int, root) -> "Twiddles":
+        if log_n >= 64:
+            check(lib.zntt_precompute_twiddles(self._h, field, log_n, _ptr(_elem(field, root)), None, log_n))
+        stages = [empty(field, 1 << s) for s in range(log_n)]
+        ptrs = (ctypes.c_void_p * max(log_n, 1))(*[s.ctypes.data for s in stages])
+        check(lib.zntt_precompute_twiddles(self._h, field, log_n, _ptr(_elem(field, root)),
+                                           ctypes.cast(ptrs, ctypes.POINTER(ctypes.c_void_p)), log_n))
+        return Twiddles(self, field, stages, registered=True)
+
+    def free_twiddles(self, tw: "Twiddles") -> None:
+        tw.free()
+
+    def ntt_with_twiddles(self, field: int, data: np.ndarray, log_n: int, twiddles) -> None:
+        self._twiddled(lib.zntt_ntt_with_twiddles, field, data, log_n, twiddles)
+
+    def intt_with_twiddles(self, field: int, data: np.ndarray, log_n: int, twiddles) -> None:
+        self._twiddled(lib.zntt_intt_with_twiddles, field, data, log_n, twiddles)
+
+    def _twiddled(self, fn, field, data, log_n, twiddles):
+        stages = twiddles.stages if isinstance(twiddles, Twiddles) else [np.ascontiguousarray(s) for s in twiddles]
+        k = len(stages)
+        ptrs = (ctypes.c_void_p * max(k, 1))(*[s.ctypes.data for s in stages])
+        lens = (ctypes.c_size_t * max(k, 1))(*[s.shape[0] for s in stages])
+        check(fn(self._h, field, _ptr(data), data.shape[0], log_n,
+                 ctypes.cast(ptrs, ctypes.POINTER(ctypes.c_void_p)), lens, k))
+
+    # ---- beyond zig-ntt ----------------------------------------------------
+    def coset_ntt(self, field: int, data: np.ndarray, log_n: int, root, shift) -> None:
+        check(lib.zntt_coset_ntt(self._h, field, _ptr(data), data.shape[0], log_n,
+                                 _ptr(_elem(field, root)), _ptr(_elem(field, shift))))
+
+    def lde(self, field: int, coeffs: np.ndarray, log_k: int, log_blowup: int, shift) -> np.ndarray:
+        out = empty(field, 1 << (log_k + log_blowup))
+        check(lib.zntt_lde(self._h, field, _ptr(coeffs), log_k, log_blowup, _ptr(_elem(field, shift)), _ptr(out)))
+        return out
+
+    def execute_device(self, field: int, direction: int, log_n: int, data_ptr: int, root, shift=None,
+                       stream: int | None = None) -> None:
+        """Transform 2^log_n elements already in device memory at data_ptr,
+        enqueued on `stream` (a hipStream_t as int; None = the context's)."""
+        r = _elem(field, root)
+        s = None if shift is None else _elem(field, shift)
+        t = Transform(field, direction, log_n, data_ptr, r.ctypes.data, None if s is None else s.ctypes.data)
+        check(lib.zntt_execute_device(self._h, ctypes.byref(t), stream))
+
+    def debug_field_op(self, field: int, op: int, a: np.ndarray, b: np.ndarray) -> np.ndarray:
+        out = np.zeros_like(a)
+        check(lib.zntt_debug_field_op(self._h, field, op, _ptr(a), _ptr(b), _ptr(out), a.shape[0]))
+        return out
+
+    def probe_peaks(self) -> tuple[float, float]:
+        mad, copy = ctypes.c_double(), ctypes.c_double()
+        check(lib.zntt_probe_peaks(self._h, ctypes.byref(mad), ctypes.byref(copy)))
+        return mad.value, copy.value
+
+
+class Twiddles:
+    """The zig `[]const []const F` table: host stages plus, when registered,
+    the device-resident copy held by the context until free()."""
+
+    def __init__(self, ctx: Context, field: int, stages: list[np.ndarray], registered: bool):
+        self.ctx, self.field, self.stages, self.registered = ctx, field, stages, registered
+
+    def free(self) -> None:
+        if self.registered and self.ctx._h:
+            k = len(self.stages)
+            ptrs = (ctypes.c_void_p * max(k, 1))(*[s.ctypes.data for s in self.stages])
+            check(lib.zntt_free_twiddles(self.ctx._h, ctypes.cast(ptrs, ctypes.POINTER(ctypes.c_void_p)), k))
+            self.registered = False
+
+    def __len__(self):
+        return len(self.stages)
+
+    def __getitem__(self, i):
+        return self.stages[i]
