@@ -164,8 +164,9 @@ int zntt_lde(zntt_ctx *ctx, int field, const void *coeffs, size_t log_k, size_t 
  * field's two-adicity. */
 int zntt_primitive_root(int field, size_t log_size, void *out);
 
-/* One transform on device-resident data (HBM), enqueued on `stream`
- * (a hipStream_t; NULL = the context's own stream).  Asynchronous: returns
+/* One transform on device-resident data (HBM), enqueued on `stream`, a
+ * hipStream_t with HIP's meaning (NULL = the null stream; zntt_ctx_stream()
+ * gives the context's own stream).  Asynchronous: returns
  * once the work is enqueued.  `root`/`shift` are host pointers to one
  * element (shift only for ZNTT_COSET_FORWARD).  The twiddle table comes from
  * the device-resident cache (built on first use, on `stream`). */
@@ -182,6 +183,9 @@ int zntt_execute_device(zntt_ctx *ctx, const zntt_transform *t, void *stream);
 
 /* Blocks until every call enqueued on the context's own stream is done. */
 int zntt_ctx_synchronize(zntt_ctx *ctx);
+
+/* The context's own HIP stream (a hipStream_t). */
+void *zntt_ctx_stream(zntt_ctx *ctx);
 
 /* ------------------------------------------------------------------------ */
 /* Diagnostics for tests and the benchmark.                                  */

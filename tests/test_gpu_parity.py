@@ -304,6 +304,16 @@ def test_c2_large_fr_against_oracle_digests(Z, ctx, log_n):
     assert np.array_equal(fwd, x)
 
 
+def test_single_hip_runtime_in_process(Z, ctx):
+    """The library and torch must share one HIP runtime (torch bundles its own)."""
+    import torch
+    assert torch.cuda.is_available()
+    maps = open("/proc/self/maps").read().splitlines()
+    hips = sorted({ln.split()[-1] for ln in maps if "libamdhip64" in ln})
+    say("HIP runtimes mapped:", hips)
+    assert len(hips) == 1, hips
+
+
 def test_device_resident_execute(Z, ctx):
     """zntt_execute_device on torch-allocated HBM, on torch's stream."""
     import torch

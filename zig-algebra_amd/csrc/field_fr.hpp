@@ -60,8 +60,15 @@ __device__ __forceinline__ uint32_t fr_opaque(uint32_t x) {
   return x;
 }
 
+// Pins the accumulator to one register pair between multiply-adds.  Without
+// it the compiler splits each column into parallel partial sums and merges
+// them with extra 64-bit adds (one more VALU op per column); a single chain
+// is cheaper because the butterfly already carries enough independent work.
+__device__ __forceinline__ void fr_chain(uint64_t &acc) { asm volatile("" : "+v"(acc)); }
+
 // Montgomery product a * b * 2^-261 mod r (result < 2r, carry-normalised).
 // a: limbs < 3 * 2^30, value < 2^261.   b: carry-normalised, value < r.
+// Column bound: 9 * 3*2^30 * 2^29 + 9 * 2^58 + 2^36 < 2^64.
 __device__ __forceinline__ Fr9 fr_mul(const Fr9 &a, const Fr9 &b) {
   constexpr uint32_t P[9] = ZNTT_FR_P29;
   const uint32_t p0 = fr_opaque(P[0]);
@@ -70,9 +77,15 @@ __device__ __forceinline__ Fr9 fr_mul(const Fr9 &a, const Fr9 &b) {
 #pragma unroll
   for (int k = 0; k < 9; k++) {
 #pragma unroll
-    for (int i = 0; i <= k; i++) acc += (uint64_t)a.v[i] * b.v[k - i];
+    for (int i = 0; i <= k; i++) {
+      acc += (uint64_t)a.v[i] * b.v[k - i];
+      fr_chain(acc);
+    }
 #pragma unroll
-    for (int i = 0; i < k; i++) acc += (uint64_t)m[i] * P[k - i];
+    for (int i = 0; i < k; i++) {
+      acc += (uint64_t)m[i] * P[k - i];
+      fr_chain(acc);
+    }
     m[k] = ((uint32_t)acc * kFrN0) & kM29;
     acc += (uint64_t)m[k] * p0;
     acc >>= 29;
@@ -83,7 +96,9 @@ __device__ __forceinline__ Fr9 fr_mul(const Fr9 &a, const Fr9 &b) {
 #pragma unroll
     for (int i = k - 8; i < 9; i++) {
       acc += (uint64_t)a.v[i] * b.v[k - i];
+      fr_chain(acc);
       acc += (uint64_t)m[i] * P[k - i];
+      fr_chain(acc);
     }
     o.v[k - 9] = (uint32_t)acc & kM29;
     acc >>= 29;
@@ -102,10 +117,14 @@ __device__ __forceinline__ void fr_norm(Fr9 &x) {
 }
 
 // Radix-2 DIT butterfly with lazy reduction: (u, v) <- (u + w v, u - w v).
+// With NORM the u input is carry-normalised first.  Output limbs are below
+// (u-limb bound) + 2^30, so normalising at every other stage keeps every limb
+// that reaches fr_mul below 2.5 * 2^30 (< 3 * 2^30).
+template <bool NORM = true>
 __device__ __forceinline__ void fr_butterfly(Fr9 &u, Fr9 &v, const Fr9 &w) {
   constexpr uint32_t C[9] = ZNTT_FR_C4R;
   Fr9 t = fr_mul(v, w);
-  fr_norm(u);
+  if (NORM) fr_norm(u);
 #pragma unroll
   for (int i = 0; i < 9; i++) {
     uint32_t ui = u.v[i];

@@ -1,0 +1,113 @@
+// Field traits shared by the generic NTT kernels: element type, the user
+// (reference) and device layouts, LDS plane access and the butterfly.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "field_fr.hpp"
+#include "field_gl.hpp"
+
+namespace zntt {
+
+// ---------------------------------------------------------------------------
+// Field traits for the generic kernels.
+// ---------------------------------------------------------------------------
+struct FrT {
+  using E = Fr9;
+  static constexpr int W = 9;   // dev words
+  static constexpr int UW = 8;  // user words
+  __device__ static E load_user(const uint32_t *base, size_t i) {
+    const uint4 *p = reinterpret_cast<const uint4 *>(base + i * 8);
+    const uint4 a = p[0], b = p[1];
+    const uint32_t w[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+    return fr_unpack8(w);
+  }
+  __device__ static void store_user(uint32_t *base, size_t i, E x) {
+    fr_canon(x);
+    uint32_t w[8];
+    fr_pack8(x, w);
+    uint4 *p = reinterpret_cast<uint4 *>(base + i * 8);
+    p[0] = make_uint4(w[0], w[1], w[2], w[3]);
+    p[1] = make_uint4(w[4], w[5], w[6], w[7]);
+  }
+  __device__ static E load_dev(const uint32_t *base, size_t i) {
+    E x;
+    const uint32_t *p = base + i * 9;
+#pragma unroll
+    for (int k = 0; k < 9; k++) x.v[k] = p[k];
+    return x;
+  }
+  __device__ static void store_dev(uint32_t *base, size_t i, const E &x) {
+    uint32_t *p = base + i * 9;
+#pragma unroll
+    for (int k = 0; k < 9; k++) p[k] = x.v[k];
+  }
+  __device__ static E lds_get(const uint32_t *lds, int plane, int i) {
+    E x;
+#pragma unroll
+    for (int k = 0; k < 9; k++) x.v[k] = lds[k * plane + i];
+    return x;
+  }
+  __device__ static void lds_put(uint32_t *lds, int plane, int i, const E &x) {
+#pragma unroll
+    for (int k = 0; k < 9; k++) lds[k * plane + i] = x.v[k];
+  }
+  __device__ static E from_words(const uint32_t *w) {
+    E x;
+#pragma unroll
+    for (int k = 0; k < 9; k++) x.v[k] = w[k];
+    return x;
+  }
+  __device__ static E mul(const E &a, const E &b) { return fr_mul(a, b); }
+  template <bool NORM = true>
+  __device__ static void bfly(E &u, E &v, const E &w) {
+    fr_butterfly<NORM>(u, v, w);
+  }
+  // Canonical dev-form constant (for tables).
+  __device__ static E canon(E x) {
+    fr_canon(x);
+    return x;
+  }
+  __device__ static E neg(const E &x) { return fr_neg_canon(x); }
+  __device__ static E add_lazy(const E &a, const E &b) {
+    E o;
+#pragma unroll
+    for (int k = 0; k < 9; k++) o.v[k] = a.v[k] + b.v[k];
+    return o;
+  }
+};
+
+struct GlT {
+  using E = uint64_t;
+  static constexpr int W = 2;
+  static constexpr int UW = 2;
+  __device__ static E load_user(const uint32_t *base, size_t i) {
+    return reinterpret_cast<const uint64_t *>(base)[i];
+  }
+  __device__ static void store_user(uint32_t *base, size_t i, E x) {
+    reinterpret_cast<uint64_t *>(base)[i] = x;
+  }
+  __device__ static E load_dev(const uint32_t *base, size_t i) { return load_user(base, i); }
+  __device__ static void store_dev(uint32_t *base, size_t i, const E &x) { store_user(base, i, x); }
+  __device__ static E lds_get(const uint32_t *lds, int plane, int i) {
+    return (uint64_t)lds[i] | ((uint64_t)lds[plane + i] << 32);
+  }
+  __device__ static void lds_put(uint32_t *lds, int plane, int i, const E &x) {
+    lds[i] = (uint32_t)x;
+    lds[plane + i] = (uint32_t)(x >> 32);
+  }
+  __device__ static E from_words(const uint32_t *w) { return (uint64_t)w[0] | ((uint64_t)w[1] << 32); }
+  __device__ static E mul(const E &a, const E &b) { return gl_mul(a, b); }
+  template <bool NORM = true>
+  __device__ static void bfly(E &u, E &v, const E &w) {
+    gl_butterfly(u, v, w);
+  }
+  __device__ static E canon(E x) { return gl_canon(x); }
+  __device__ static E neg(const E &x) { return gl_neg(x); }
+};
+
+__device__ __forceinline__ uint32_t brev_bits(uint32_t x, int bits) {
+  return bits == 0 ? 0u : (__brev(x) >> (32 - bits));
+}
+
+}  // namespace zntt

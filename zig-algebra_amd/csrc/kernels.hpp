@@ -42,6 +42,12 @@ PassPlan plan_passes(int field, int L);
 hipError_t launch_transform(int field, int L, uint32_t *data, uint32_t *scratch, const uint32_t *tw,
                             const uint32_t *coset, const uint32_t *scale_dev, hipStream_t stream);
 
+// Per-field pass launchers (pass_fr.hip, pass_gl.hip).
+hipError_t launch_transform_fr(int L, uint32_t *data, uint32_t *scratch, const uint32_t *tw, const uint32_t *coset,
+                               const uint32_t *scale_dev, hipStream_t stream);
+hipError_t launch_transform_gl(int L, uint32_t *data, uint32_t *scratch, const uint32_t *tw, const uint32_t *coset,
+                               const uint32_t *scale_dev, hipStream_t stream);
+
 // Twiddle table builders (dev layout).  `flat` holds 2^L entries: stages
 // 1..L at offsets 2^(s-1)-1 and a trailing one (index 2^L - 1).
 //   pows: dev-form root^(2^k), k < L (device buffer, L entries)

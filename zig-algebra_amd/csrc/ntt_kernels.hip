@@ -25,186 +25,10 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#include "field_fr.hpp"
-#include "field_gl.hpp"
+#include "field_traits.hpp"
 #include "kernels.hpp"
 
 namespace zntt {
-
-// ---------------------------------------------------------------------------
-// Field traits for the generic kernels.
-// ---------------------------------------------------------------------------
-struct FrT {
-  using E = Fr9;
-  static constexpr int W = 9;   // dev words
-  static constexpr int UW = 8;  // user words
-  __device__ static E load_user(const uint32_t *base, size_t i) {
-    const uint4 *p = reinterpret_cast<const uint4 *>(base + i * 8);
-    const uint4 a = p[0], b = p[1];
-    const uint32_t w[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
-    return fr_unpack8(w);
-  }
-  __device__ static void store_user(uint32_t *base, size_t i, E x) {
-    fr_canon(x);
-    uint32_t w[8];
-    fr_pack8(x, w);
-    uint4 *p = reinterpret_cast<uint4 *>(base + i * 8);
-    p[0] = make_uint4(w[0], w[1], w[2], w[3]);
-    p[1] = make_uint4(w[4], w[5], w[6], w[7]);
-  }
-  __device__ static E load_dev(const uint32_t *base, size_t i) {
-    E x;
-    const uint32_t *p = base + i * 9;
-#pragma unroll
-    for (int k = 0; k < 9; k++) x.v[k] = p[k];
-    return x;
-  }
-  __device__ static void store_dev(uint32_t *base, size_t i, const E &x) {
-    uint32_t *p = base + i * 9;
-#pragma unroll
-    for (int k = 0; k < 9; k++) p[k] = x.v[k];
-  }
-  __device__ static E lds_get(const uint32_t *lds, int plane, int i) {
-    E x;
-#pragma unroll
-    for (int k = 0; k < 9; k++) x.v[k] = lds[k * plane + i];
-    return x;
-  }
-  __device__ static void lds_put(uint32_t *lds, int plane, int i, const E &x) {
-#pragma unroll
-    for (int k = 0; k < 9; k++) lds[k * plane + i] = x.v[k];
-  }
-  __device__ static E from_words(const uint32_t *w) {
-    E x;
-#pragma unroll
-    for (int k = 0; k < 9; k++) x.v[k] = w[k];
-    return x;
-  }
-  __device__ static E mul(const E &a, const E &b) { return fr_mul(a, b); }
-  __device__ static void bfly(E &u, E &v, const E &w) { fr_butterfly(u, v, w); }
-  // Canonical dev-form constant (for tables).
-  __device__ static E canon(E x) {
-    fr_canon(x);
-    return x;
-  }
-  __device__ static E neg(const E &x) { return fr_neg_canon(x); }
-  __device__ static E add_lazy(const E &a, const E &b) {
-    E o;
-#pragma unroll
-    for (int k = 0; k < 9; k++) o.v[k] = a.v[k] + b.v[k];
-    return o;
-  }
-};
-
-struct GlT {
-  using E = uint64_t;
-  static constexpr int W = 2;
-  static constexpr int UW = 2;
-  __device__ static E load_user(const uint32_t *base, size_t i) {
-    return reinterpret_cast<const uint64_t *>(base)[i];
-  }
-  __device__ static void store_user(uint32_t *base, size_t i, E x) {
-    reinterpret_cast<uint64_t *>(base)[i] = x;
-  }
-  __device__ static E load_dev(const uint32_t *base, size_t i) { return load_user(base, i); }
-  __device__ static void store_dev(uint32_t *base, size_t i, const E &x) { store_user(base, i, x); }
-  __device__ static E lds_get(const uint32_t *lds, int plane, int i) {
-    return (uint64_t)lds[i] | ((uint64_t)lds[plane + i] << 32);
-  }
-  __device__ static void lds_put(uint32_t *lds, int plane, int i, const E &x) {
-    lds[i] = (uint32_t)x;
-    lds[plane + i] = (uint32_t)(x >> 32);
-  }
-  __device__ static E from_words(const uint32_t *w) { return (uint64_t)w[0] | ((uint64_t)w[1] << 32); }
-  __device__ static E mul(const E &a, const E &b) { return gl_mul(a, b); }
-  __device__ static void bfly(E &u, E &v, const E &w) { gl_butterfly(u, v, w); }
-  __device__ static E canon(E x) { return gl_canon(x); }
-  __device__ static E neg(const E &x) { return gl_neg(x); }
-};
-
-__device__ __forceinline__ uint32_t brev_bits(uint32_t x, int bits) {
-  return bits == 0 ? 0u : (__brev(x) >> (32 - bits));
-}
-
-// ---------------------------------------------------------------------------
-// The stage-range pass.
-// ---------------------------------------------------------------------------
-template <class F>
-__global__ __launch_bounds__(256) void ntt_pass_kernel(const PassArgs a) {
-  using E = typename F::E;
-  extern __shared__ uint32_t lds[];
-  const int K = a.K, lg = a.log_g, G = 1 << lg;
-  const int tile = G << K;
-  const int lo_bits = a.s0 - 1;
-  const int hi_bits = a.L - lo_bits - K;
-  const size_t w = blockIdx.x;
-  size_t hi = 0, lo0 = 0, c0 = 0;
-  if (a.first) {
-    c0 = w << lg;
-  } else {
-    const int lbb = lo_bits - lg;
-    hi = w >> lbb;
-    lo0 = (w & (((size_t)1 << lbb) - 1)) << lg;
-  }
-
-  // Gather the tile: element (g, mid) of the tile sits at LDS index mid*G + g.
-  for (int e = threadIdx.x; e < tile; e += blockDim.x) {
-    const int g = e & (G - 1), mid = e >> lg;
-    E x;
-    if (a.first) {
-      const size_t src = ((size_t)brev_bits(mid, K) << hi_bits) + c0 + g;
-      x = F::load_user(a.src, src);
-      if (a.coset) x = F::mul(x, F::load_dev(a.coset, src));
-    } else {
-      const size_t pos = (hi << (lo_bits + K)) + ((size_t)mid << lo_bits) + lo0 + g;
-      x = F::load_dev(a.src, pos);
-    }
-    F::lds_put(lds, tile, e, x);
-  }
-  __syncthreads();
-
-  for (int l = 0; l < K; l++) {
-    const int s = a.s0 + l;
-    const int half = 1 << l;
-    const size_t stage_off = ((size_t)1 << (s - 1)) - 1;
-    for (int b = threadIdx.x; b < (tile >> 1); b += blockDim.x) {
-      const int g = b & (G - 1), q = b >> lg;
-      const int jl = q & (half - 1), blk = q >> l;
-      const int mid0 = (blk << (l + 1)) | jl;
-      const size_t lo = a.first ? 0 : lo0 + g;
-      const E wv = F::load_dev(a.tw, stage_off + ((size_t)jl << lo_bits) + lo);
-      const int i0 = mid0 * G + g, i1 = i0 + (half << lg);
-      E u = F::lds_get(lds, tile, i0), v = F::lds_get(lds, tile, i1);
-      F::bfly(u, v, wv);
-      F::lds_put(lds, tile, i0, u);
-      F::lds_put(lds, tile, i1, v);
-    }
-    __syncthreads();
-  }
-
-  E scale;
-  if (a.last && a.has_scale) scale = F::from_words(a.scale);
-  for (int e = threadIdx.x; e < tile; e += blockDim.x) {
-    int g, mid;
-    size_t pos;
-    if (a.first) {
-      g = e >> K;
-      mid = e & ((1 << K) - 1);
-      pos = ((size_t)brev_bits((uint32_t)(c0 + g), hi_bits) << K) + mid;
-    } else {
-      g = e & (G - 1);
-      mid = e >> lg;
-      pos = (hi << (lo_bits + K)) + ((size_t)mid << lo_bits) + lo0 + g;
-    }
-    E x = F::lds_get(lds, tile, mid * G + g);
-    if (a.last) {
-      if (a.has_scale) x = F::mul(x, scale);
-      F::store_user(a.dst, pos, x);
-    } else {
-      F::store_dev(a.dst, pos, x);
-    }
-  }
-}
 
 static int tile_log_for(int field) { return field == kFr ? 11 : 12; }
 
@@ -233,58 +57,11 @@ PassPlan plan_passes(int field, int L) {
   return p;
 }
 
-template <class F>
-static hipError_t run_passes(int field, int L, uint32_t *data, uint32_t *scratch, const uint32_t *tw,
-                             const uint32_t *coset, const uint32_t *scale_dev, hipStream_t stream) {
-  const PassPlan plan = plan_passes(field, L);
-  int s0 = 1;
-  for (int i = 0; i < plan.passes; i++) {
-    PassArgs a = {};
-    a.first = i == 0;
-    a.last = i == plan.passes - 1;
-    a.src = a.first ? data : scratch;
-    a.dst = a.last ? data : scratch;
-    a.tw = tw;
-    a.coset = a.first ? coset : nullptr;
-    a.has_scale = (a.last && scale_dev) ? 1 : 0;
-    if (a.has_scale)
-      for (int k = 0; k < F::W; k++) a.scale[k] = scale_dev[k];
-    a.L = L;
-    a.s0 = s0;
-    a.K = plan.K[i];
-    a.log_g = plan.log_g[i];
-    const size_t tile = (size_t)1 << (a.K + a.log_g);
-    const size_t blocks = ((size_t)1 << L) / tile;
-    const size_t lds = (size_t)F::W * 4 * tile;
-    const int threads = tile >= 512 ? 256 : (int)((tile / 2) < 64 ? 64 : tile / 2);
-    hipLaunchKernelGGL(ntt_pass_kernel<F>, dim3((unsigned)blocks), dim3(threads), lds, stream, a);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    s0 += a.K;
-  }
-  return hipSuccess;
-}
-
-static bool lds_configured = false;
-static hipError_t configure_lds() {
-  if (lds_configured) return hipSuccess;
-  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&ntt_pass_kernel<FrT>),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, 9 * 4 * (1 << 11));
-  if (e != hipSuccess) return e;
-  e = hipFuncSetAttribute(reinterpret_cast<const void *>(&ntt_pass_kernel<GlT>),
-                          hipFuncAttributeMaxDynamicSharedMemorySize, 2 * 4 * (1 << 12));
-  if (e != hipSuccess) return e;
-  lds_configured = true;
-  return hipSuccess;
-}
-
 hipError_t launch_transform(int field, int L, uint32_t *data, uint32_t *scratch, const uint32_t *tw,
                             const uint32_t *coset, const uint32_t *scale_dev, hipStream_t stream) {
   if (L == 0) return hipSuccess;
-  hipError_t e = configure_lds();
-  if (e != hipSuccess) return e;
-  if (field == kFr) return run_passes<FrT>(field, L, data, scratch, tw, coset, scale_dev, stream);
-  return run_passes<GlT>(field, L, data, scratch, tw, coset, scale_dev, stream);
+  if (field == kFr) return launch_transform_fr(L, data, scratch, tw, coset, scale_dev, stream);
+  return launch_transform_gl(L, data, scratch, tw, coset, scale_dev, stream);
 }
 
 // ---------------------------------------------------------------------------

@@ -1,0 +1,343 @@
+// The stage-range pass kernels (templates), instantiated per field in
+// pass_fr.hip and pass_gl.hip so the two compile in parallel.
+//
+// A pass covers DIT stages s0..s0+K-1 of a 2^L-point transform; see
+// ntt_kernels.hip for the overall schedule.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <utility>
+
+#include "field_traits.hpp"
+#include "kernels.hpp"
+
+namespace zntt {
+
+// ---------------------------------------------------------------------------
+// The stage-range pass.
+// ---------------------------------------------------------------------------
+template <class F>
+__global__ __launch_bounds__(256) void ntt_pass_kernel(const PassArgs a) {
+  using E = typename F::E;
+  extern __shared__ uint32_t lds[];
+  const int K = a.K, lg = a.log_g, G = 1 << lg;
+  const int tile = G << K;
+  const int lo_bits = a.s0 - 1;
+  const int hi_bits = a.L - lo_bits - K;
+  const size_t w = blockIdx.x;
+  size_t hi = 0, lo0 = 0, c0 = 0;
+  if (a.first) {
+    c0 = w << lg;
+  } else {
+    const int lbb = lo_bits - lg;
+    hi = w >> lbb;
+    lo0 = (w & (((size_t)1 << lbb) - 1)) << lg;
+  }
+
+  // Gather the tile: element (g, mid) of the tile sits at LDS index mid*G + g.
+  for (int e = threadIdx.x; e < tile; e += blockDim.x) {
+    const int g = e & (G - 1), mid = e >> lg;
+    E x;
+    if (a.first) {
+      const size_t src = ((size_t)brev_bits(mid, K) << hi_bits) + c0 + g;
+      x = F::load_user(a.src, src);
+      if (a.coset) x = F::mul(x, F::load_dev(a.coset, src));
+    } else {
+      const size_t pos = (hi << (lo_bits + K)) + ((size_t)mid << lo_bits) + lo0 + g;
+      x = F::load_dev(a.src, pos);
+    }
+    F::lds_put(lds, tile, e, x);
+  }
+  __syncthreads();
+
+  for (int l = 0; l < K; l++) {
+    const int s = a.s0 + l;
+    const int half = 1 << l;
+    const size_t stage_off = ((size_t)1 << (s - 1)) - 1;
+    for (int b = threadIdx.x; b < (tile >> 1); b += blockDim.x) {
+      const int g = b & (G - 1), q = b >> lg;
+      const int jl = q & (half - 1), blk = q >> l;
+      const int mid0 = (blk << (l + 1)) | jl;
+      const size_t lo = a.first ? 0 : lo0 + g;
+      const E wv = F::load_dev(a.tw, stage_off + ((size_t)jl << lo_bits) + lo);
+      const int i0 = mid0 * G + g, i1 = i0 + (half << lg);
+      E u = F::lds_get(lds, tile, i0), v = F::lds_get(lds, tile, i1);
+      F::template bfly<true>(u, v, wv);
+      F::lds_put(lds, tile, i0, u);
+      F::lds_put(lds, tile, i1, v);
+    }
+    __syncthreads();
+  }
+
+  E scale;
+  if (a.last && a.has_scale) scale = F::from_words(a.scale);
+  for (int e = threadIdx.x; e < tile; e += blockDim.x) {
+    int g, mid;
+    size_t pos;
+    if (a.first) {
+      g = e >> K;
+      mid = e & ((1 << K) - 1);
+      pos = ((size_t)brev_bits((uint32_t)(c0 + g), hi_bits) << K) + mid;
+    } else {
+      g = e & (G - 1);
+      mid = e >> lg;
+      pos = (hi << (lo_bits + K)) + ((size_t)mid << lo_bits) + lo0 + g;
+    }
+    E x = F::lds_get(lds, tile, mid * G + g);
+    if (a.last) {
+      if (a.has_scale) x = F::mul(x, scale);
+      F::store_user(a.dst, pos, x);
+    } else {
+      F::store_dev(a.dst, pos, x);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// The stage-range pass with register rounds (tiles of >= 512 elements).
+//
+// Same tile and DAG as ntt_pass_kernel, but each thread keeps 8 elements in
+// registers and runs up to three stages on them between LDS exchanges:
+// a "set" is 2^R elements whose tile positions differ only in mid bits
+// [l0, l0+R), which is exactly what stages l0..l0+R-1 couple.  The first
+// round reads HBM directly and the last round writes HBM directly, so a pass
+// of K stages costs ceil(K/3) - 1 LDS exchanges and barriers instead of K.
+// ---------------------------------------------------------------------------
+
+// XOR bits [3,5) of a tile index with bits [6,8): the first round's lanes run
+// over 8 consecutive groups and then jump by whole 64-word rows; the swizzle
+// sends those runs to distinct LDS banks.  A bijection on any tile >= 256.
+__device__ __forceinline__ int lds_swz(int i) { return i ^ (((i >> 6) & 3) << 3); }
+
+struct TileCtx {
+  int K, lg, G, tile, lo_bits, hi_bits;
+  size_t hi, lo0, c0;
+};
+
+// Compile-time loop: f(integral_constant<int, 0>) ... f(integral_constant<int, N-1>).
+// Every index into the register arrays below is a constant expression, so the
+// arrays are scalarised into VGPRs however large the unrolled body gets.
+template <class Fn, int... Is>
+__device__ __forceinline__ void sfor_impl(Fn &&f, std::integer_sequence<int, Is...>) {
+  (f(std::integral_constant<int, Is>{}), ...);
+}
+template <int N, class Fn>
+__device__ __forceinline__ void sfor(Fn &&f) {
+  sfor_impl(f, std::make_integer_sequence<int, N>{});
+}
+
+// ODD: parity of the pass's first stage s0, so that the normalise-at-odd-stage
+// rule folds to a constant and each round is one straight-line block.
+template <class F, int R, int l0, bool first_round, bool last_round, bool ODD>
+__device__ __forceinline__ void pass_round(typename F::E (&x)[8], const PassArgs &a, uint32_t *lds,
+                                           const TileCtx &c) {
+  using E = typename F::E;
+  constexpr int SET = 1 << R, NSETS = 8 >> R;
+  int g[NSETS], base[NSETS];
+  sfor<NSETS>([&](auto i) {
+    const int slot = threadIdx.x + i * blockDim.x;
+    g[i] = slot & (c.G - 1);
+    const int rest = slot >> c.lg;
+    base[i] = (rest & ((1 << l0) - 1)) | ((rest >> l0) << (l0 + R));
+  });
+  // Twiddles: stage l0+t couples (k, k + 2^t) with index jl = mid mod 2^(l0+t),
+  // so a set needs 2^t of them at stage t; issue all loads before computing.
+  E tw[NSETS * (SET - 1)];
+  sfor<R>([&](auto t) {
+    const int s = a.s0 + l0 + t;
+    const size_t off = ((size_t)1 << (s - 1)) - 1;
+    sfor<NSETS>([&](auto i) {
+      const size_t lo = a.first ? 0 : c.lo0 + g[i];
+      const int jbase = base[i] & ((1 << l0) - 1);
+      sfor<(1 << t)>([&](auto kk) {
+        tw[i * (SET - 1) + (1 << t) - 1 + kk] =
+            F::load_dev(a.tw, off + ((size_t)(jbase | (kk << l0)) << c.lo_bits) + lo);
+      });
+    });
+  });
+  // Load the sets.
+  sfor<NSETS>([&](auto i) {
+    sfor<SET>([&](auto k) {
+      const int mid = base[i] | (k << l0);
+      if constexpr (first_round) {
+        if (a.first) {
+          const size_t src = ((size_t)brev_bits(mid, c.K) << c.hi_bits) + c.c0 + g[i];
+          x[i * SET + k] = F::load_user(a.src, src);
+        } else {
+          const size_t pos = (c.hi << (c.lo_bits + c.K)) + ((size_t)mid << c.lo_bits) + c.lo0 + g[i];
+          x[i * SET + k] = F::load_dev(a.src, pos);
+        }
+      } else {
+        x[i * SET + k] = F::lds_get(lds, c.tile, lds_swz(mid * c.G + g[i]));
+      }
+    });
+  });
+  if constexpr (first_round) {
+    if (a.first && a.coset) {  // coset pre-scale: data[i] *= shift^i at the source index
+      sfor<NSETS>([&](auto i) {
+        sfor<SET>([&](auto k) {
+          const int mid = base[i] | (k << l0);
+          const size_t src = ((size_t)brev_bits(mid, c.K) << c.hi_bits) + c.c0 + g[i];
+          x[i * SET + k] = F::mul(x[i * SET + k], F::load_dev(a.coset, src));
+        });
+      });
+    }
+  }
+  // R radix-2 stages in registers; carry-normalise u at odd stages.
+  sfor<R>([&](auto t) {
+    constexpr bool norm = (((ODD ? 1 : 0) + l0 + decltype(t)::value) & 1) != 0;
+    sfor<NSETS>([&](auto i) {
+      sfor<SET>([&](auto k) {
+        if constexpr ((decltype(k)::value & (1 << decltype(t)::value)) == 0) {
+          constexpr int T = decltype(t)::value, KK = decltype(k)::value;
+          F::template bfly<norm>(x[i * SET + KK], x[i * SET + KK + (1 << T)],
+                                 tw[i * (SET - 1) + (1 << T) - 1 + (KK & ((1 << T) - 1))]);
+        }
+      });
+    });
+  });
+  // Store the sets (to the positions they came from, or to HBM).
+  if constexpr (last_round) {
+    if (a.last && a.has_scale) {
+      const E scale = F::from_words(a.scale);
+      sfor<8>([&](auto j) { x[j] = F::mul(x[j], scale); });
+    }
+  }
+  sfor<NSETS>([&](auto i) {
+    sfor<SET>([&](auto k) {
+      const int mid = base[i] | (k << l0);
+      if constexpr (last_round) {
+        size_t pos;
+        if (a.first)
+          pos = ((size_t)brev_bits((uint32_t)(c.c0 + g[i]), c.hi_bits) << c.K) + mid;
+        else
+          pos = (c.hi << (c.lo_bits + c.K)) + ((size_t)mid << c.lo_bits) + c.lo0 + g[i];
+        if (a.last)
+          F::store_user(a.dst, pos, x[i * SET + k]);
+        else
+          F::store_dev(a.dst, pos, x[i * SET + k]);
+      } else {
+        F::lds_put(lds, c.tile, lds_swz(mid * c.G + g[i]), x[i * SET + k]);
+      }
+    });
+  });
+}
+
+// Rounds of a K-stage pass: the first takes K - 3*(ceil(K/3) - 1) stages, the
+// rest three each; unrolled at compile time so `x` stays in registers.
+template <class F, int K, int RI, bool ODD>
+__device__ __forceinline__ void pass_rounds(typename F::E (&x)[8], const PassArgs &a, uint32_t *lds,
+                                            const TileCtx &c) {
+  constexpr int NR = (K + 2) / 3;
+  constexpr int R0 = K - 3 * (NR - 1);
+  constexpr int R = RI == 0 ? R0 : 3;
+  constexpr int L0 = RI == 0 ? 0 : R0 + 3 * (RI - 1);
+  pass_round<F, R, L0, RI == 0, RI == NR - 1, ODD>(x, a, lds, c);
+  if constexpr (RI + 1 < NR) {
+    __syncthreads();
+    pass_rounds<F, K, RI + 1, ODD>(x, a, lds, c);
+  }
+}
+
+template <class F, int K, bool ODD>
+__global__ __launch_bounds__(512) void ntt_pass_r8_kernel(const PassArgs a) {
+  using E = typename F::E;
+  extern __shared__ uint32_t lds[];
+  TileCtx c;
+  c.K = K;
+  c.lg = a.log_g;
+  c.G = 1 << c.lg;
+  c.tile = c.G << c.K;
+  c.lo_bits = a.s0 - 1;
+  c.hi_bits = a.L - c.lo_bits - c.K;
+  const size_t w = blockIdx.x;
+  c.hi = c.lo0 = c.c0 = 0;
+  if (a.first) {
+    c.c0 = w << c.lg;
+  } else {
+    const int lbb = c.lo_bits - c.lg;
+    c.hi = w >> lbb;
+    c.lo0 = (w & (((size_t)1 << lbb) - 1)) << c.lg;
+  }
+  E x[8];
+  pass_rounds<F, K, 0, ODD>(x, a, lds, c);
+}
+
+// Pass lengths the planner produces for tiles of >= 512 elements: 5..12.
+template <class F>
+static hipError_t launch_r8(int K, bool odd, dim3 grid, dim3 block, size_t lds, hipStream_t s, const PassArgs &a) {
+  switch (K * 2 + (odd ? 1 : 0)) {
+#define ZNTT_R8(k)                                                                                     \
+  case 2 * k: hipLaunchKernelGGL((ntt_pass_r8_kernel<F, k, false>), grid, block, lds, s, a); break; \
+  case 2 * k + 1: hipLaunchKernelGGL((ntt_pass_r8_kernel<F, k, true>), grid, block, lds, s, a); break;
+    ZNTT_R8(5) ZNTT_R8(6) ZNTT_R8(7) ZNTT_R8(8) ZNTT_R8(9) ZNTT_R8(10) ZNTT_R8(11) ZNTT_R8(12)
+#undef ZNTT_R8
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <class F>
+static hipError_t set_r8_lds(size_t bytes) {
+#define ZNTT_R8A(k, o)                                                                                       \
+  {                                                                                                          \
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&ntt_pass_r8_kernel<F, k, o>),        \
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);              \
+    if (e != hipSuccess) return e;                                                                           \
+  }
+#define ZNTT_R8B(k) ZNTT_R8A(k, false) ZNTT_R8A(k, true)
+  ZNTT_R8B(5) ZNTT_R8B(6) ZNTT_R8B(7) ZNTT_R8B(8) ZNTT_R8B(9) ZNTT_R8B(10) ZNTT_R8B(11) ZNTT_R8B(12)
+#undef ZNTT_R8B
+#undef ZNTT_R8A
+  return hipSuccess;
+}
+
+template <class F>
+static hipError_t run_passes_t(int field, int L, uint32_t *data, uint32_t *scratch, const uint32_t *tw,
+                             const uint32_t *coset, const uint32_t *scale_dev, hipStream_t stream) {
+  const PassPlan plan = plan_passes(field, L);
+  int s0 = 1;
+  for (int i = 0; i < plan.passes; i++) {
+    PassArgs a = {};
+    a.first = i == 0;
+    a.last = i == plan.passes - 1;
+    a.src = a.first ? data : scratch;
+    a.dst = a.last ? data : scratch;
+    a.tw = tw;
+    a.coset = a.first ? coset : nullptr;
+    a.has_scale = (a.last && scale_dev) ? 1 : 0;
+    if (a.has_scale)
+      for (int k = 0; k < F::W; k++) a.scale[k] = scale_dev[k];
+    a.L = L;
+    a.s0 = s0;
+    a.K = plan.K[i];
+    a.log_g = plan.log_g[i];
+    const size_t tile = (size_t)1 << (a.K + a.log_g);
+    const size_t blocks = ((size_t)1 << L) / tile;
+    const size_t lds = (size_t)F::W * 4 * tile;
+    if (tile >= 512 && a.K >= 5) {  // eight elements per thread, register rounds
+      hipError_t e =
+          launch_r8<F>(a.K, (a.s0 & 1) != 0, dim3((unsigned)blocks), dim3((unsigned)(tile / 8)), lds, stream, a);
+      if (e != hipSuccess) return e;
+    } else {
+      const int threads = (int)((tile / 2) < 64 ? 64 : tile / 2);
+      hipLaunchKernelGGL(ntt_pass_kernel<F>, dim3((unsigned)blocks), dim3(threads), lds, stream, a);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    s0 += a.K;
+  }
+  return hipSuccess;
+}
+
+// Raises the dynamic-LDS limit of every pass kernel of field F to `bytes`.
+template <class F>
+static hipError_t configure_pass_lds(size_t bytes) {
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&ntt_pass_kernel<F>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  if (e != hipSuccess) return e;
+  return set_r8_lds<F>(bytes);
+}
+
+}  // namespace zntt

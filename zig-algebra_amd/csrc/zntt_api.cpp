@@ -501,6 +501,8 @@ void zntt_ctx_destroy(zntt_ctx *c) {
 
 uint64_t zntt_ctx_bytes_in_use(const zntt_ctx *c) { return c ? c->in_use : 0; }
 
+void *zntt_ctx_stream(zntt_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
 int zntt_ctx_synchronize(zntt_ctx *c) {
   if (!c) return ZNTT_INVALID_ARGUMENT;
   return hip_status(hipStreamSynchronize(c->stream));
@@ -721,7 +723,7 @@ int zntt_execute_device(zntt_ctx *c, const zntt_transform *t, void *stream) {
   if (!t->data) return ZNTT_INVALID_ARGUMENT;
   if (t->direction == ZNTT_COSET_FORWARD && !t->shift) return ZNTT_INVALID_ARGUMENT;
   if (t->direction < ZNTT_FORWARD || t->direction > ZNTT_COSET_FORWARD) return ZNTT_INVALID_ARGUMENT;
-  hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+  hipStream_t s = (hipStream_t)stream;  // HIP's convention: NULL is the null stream
   CallGuard g(c);
   const int field = t->field, L = (int)t->log_n;
   const size_t n = (size_t)1 << L;

@@ -84,7 +84,7 @@ EXPORTED = [
     "zntt_element_size", "zntt_bit_reverse", "zntt_ntt", "zntt_intt", "zntt_precompute_twiddles",
     "zntt_free_twiddles", "zntt_ntt_with_twiddles", "zntt_intt_with_twiddles", "zntt_coset_ntt", "zntt_lde",
     "zntt_primitive_root", "zntt_execute_device", "zntt_ctx_synchronize", "zntt_debug_field_op",
-    "zntt_probe_peaks", "zntt_fill_uniform",
+    "zntt_probe_peaks", "zntt_fill_uniform", "zntt_ctx_stream",
 ]
 
 
@@ -96,6 +96,14 @@ class Transform(ctypes.Structure):
 def _load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is not built; run `make -C zig-algebra_amd` (or __graft_entry__.build())")
+    # PyTorch-ROCm ships its own HIP/HSA runtime.  Two HIP runtimes in one
+    # process cannot share the GPU, so when torch is installed it is loaded
+    # first and the library's libamdhip64 dependency binds to that copy; device
+    # pointers and streams from torch are then valid in the library.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(LIB_PATH)
     vp, sz, i32, u64 = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_uint64
     pp = ctypes.POINTER(ctypes.c_void_p)
@@ -121,6 +129,8 @@ def _load():
     L.zntt_primitive_root.argtypes = [i32, sz, vp]
     L.zntt_execute_device.argtypes = [vp, ctypes.POINTER(Transform), vp]
     L.zntt_ctx_synchronize.argtypes = [vp]
+    L.zntt_ctx_stream.argtypes = [vp]
+    L.zntt_ctx_stream.restype = vp
     L.zntt_debug_field_op.argtypes = [vp, i32, i32, vp, vp, vp, sz]
     L.zntt_probe_peaks.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
     L.zntt_fill_uniform.argtypes = [i32, u64, vp, sz]
@@ -250,7 +260,10 @@ class Context:
     def execute_device(self, field: int, direction: int, log_n: int, data_ptr: int, root, shift=None,
                        stream: int | None = None) -> None:
         """Transform 2^log_n elements already in device memory at data_ptr,
-        enqueued on `stream` (a hipStream_t as int; None = the context's)."""
+        enqueued on `stream` (a hipStream_t as int, 0 = HIP's null stream;
+        None = the context's own stream)."""
+        if stream is None:
+            stream = lib.zntt_ctx_stream(self._h)
         r = _elem(field, root)
         s = None if shift is None else _elem(field, shift)
         t = Transform(field, direction, log_n, data_ptr, r.ctypes.data, None if s is None else s.ctypes.data)
