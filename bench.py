@@ -1,19 +1,27 @@
 #!/usr/bin/env python3
-"""Benchmark: BN254 Fr forward NTT of 2^24 elements on one MI355X.
+"""Benchmarks of the MI355X NTT library on one GPU.
 
-A step is one in-place forward NTT (libs/ntt/src/root.zig:65-90 semantics,
-root = primitive_root(24)) over 2^24 BN254 Fr elements already resident in
-HBM, like the reference's own benchmark which transforms the same buffer in
-place on every iteration (libs/pairing/src/bench.zig:16-24, 107-111).
+Default (the headline line): BN254 Fr forward NTT of 2^24 elements.  A step
+is one in-place forward NTT (libs/ntt/src/root.zig:65-90 semantics, root =
+primitive_root(24)) over 2^24 elements already resident in HBM, like the
+reference's own benchmark, which transforms the same buffer in place on every
+iteration (libs/pairing/src/bench.zig:16-24, 107-111).
 
-Prints one JSON line:
+Other configurations print their own JSON line when asked for:
+  --workload gl-batch   4096 Goldilocks transforms of 2^12, one persistent launch
+  --workload fr-uneven  512 BN254 Fr transforms, 2^10..2^20, forward / inverse /
+                        coset-forward, one persistent launch vs one call each
+  --workload serve      the LDE serving loop (zig-algebra_amd/zntt_amd/serve.py)
+
+The JSON line carries:
   value        whole-job throughput, Melem/s (elements transformed / time)
-  roofline     the dominant kernel (the LDS stage-range pass) against the
-               VALU 32-bit multiply peak: 136 32-bit multiplies per 254-bit
-               Montgomery product (2 s^2 + s for s = 8 limbs), one product
-               per butterfly, (n/2) * log2 n butterflies per transform
-  cpu_baseline the CPU oracle (a C restatement of the reference path) on the
-               same workload, one thread, timed on this host
+  roofline     the dominant kernel against its bound: for BN254 Fr the VALU
+               32-bit multiply peak (136 32-bit multiplies per 254-bit
+               Montgomery product, one product per butterfly, (n/2) log2 n
+               butterflies); for the Goldilocks batch HBM bandwidth (16 bytes
+               per element: one read, one write)
+  cpu_baseline the CPU oracle (a C restatement of the reference path) on a
+               bounded sample of the same workload, timed on this host
 """
 import argparse
 import hashlib
@@ -29,34 +37,50 @@ MI355X_CUS = 256
 MI355X_CLOCK_HZ = 2.4e9
 LANES_PER_CU_CLK = 64          # v_mad_u64_u32 issues at full VALU rate on gfx950
 VALU_MUL_PEAK = MI355X_CUS * MI355X_CLOCK_HZ * LANES_PER_CU_CLK  # 3.93e13 mul/s
+HBM_PEAK = 8.0e12              # B/s, MI355X HBM3E
 MULS_PER_MODMUL = 136
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "fr_ntt_2p24_traffic.json")
+GL_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "gl_batch_4096x2p12_traffic.json")
 
 
-def main() -> int:
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=30)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--log-n", type=int, default=24)
-    ap.add_argument("--cpu-log-n", type=int, default=24, help="size of the CPU-baseline sample")
-    ap.add_argument("--skip-cpu", action="store_true")
-    ap.add_argument("--skip-host", action="store_true", help="skip the PCIe-inclusive host-buffer timing")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              "(transforms shard with no communication, see DESIGN.md)", file=sys.stderr)
-        return 2
+def fr_passes(log_n: int) -> int:
+    """Launches per BN254 Fr transform: one pass up to 2^11, else ceil(log_n / 9)."""
+    return 1 if log_n <= 11 else -(-log_n // 9)
 
+
+def read_traffic(path):
+    try:
+        return json.load(open(path)).get("hbm_bytes_per_launch")
+    except (ValueError, OSError):
+        return None
+
+
+def timed(step, steps, warmup, stream):
+    import torch
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    for _ in range(steps):
+        step()
+    end.record(stream)
+    torch.cuda.synchronize()
+    return start.elapsed_time(end) / steps
+
+
+def base_line(metric, value, unit, args, ms_step, config, data, dtype):
+    return {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": dtype, "data": data, "config": config}
+
+
+def run_fr_ntt(args, Z, ctx):
     import numpy as np
     import torch
-
-    import zntt_amd as Z
-
     log_n = args.log_n
     n = 1 << log_n
     field = Z.FIELD_FR
-    ctx = Z.Context(0)
     t0 = time.time()
     x = Z.fill_uniform(field, 1, n)
     root = Z.primitive_root(field, log_n)
@@ -64,25 +88,12 @@ def main() -> int:
     dev = torch.from_numpy(x.view(np.int64)).cuda()
     stream = torch.cuda.current_stream()
     sp = stream.cuda_stream
-
-    def step():
-        ctx.execute_device(field, Z.FORWARD, log_n, dev.data_ptr(), root, stream=sp)
-
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    start.record(stream)
-    for _ in range(args.steps):
-        step()
-    end.record(stream)
-    torch.cuda.synchronize()
-    ms_total = start.elapsed_time(end)
-    ms_step = ms_total / args.steps
+    ms_step = timed(lambda: ctx.execute_device(field, Z.FORWARD, log_n, dev.data_ptr(), root, stream=sp),
+                    args.steps, args.warmup, stream)
     value = n / (ms_step * 1e-3) / 1e6
 
-    # Correctness of the benchmarked path: one fresh forward transform of the
-    # seeded input, compared with the committed digest of the oracle's output.
+    # Correctness of the benchmarked path: a fresh forward transform of the
+    # seeded input against the committed digest of the oracle's output.
     check = torch.from_numpy(x.view(np.int64)).cuda()
     ctx.execute_device(field, Z.FORWARD, log_n, check.data_ptr(), root, stream=sp)
     torch.cuda.synchronize()
@@ -91,28 +102,18 @@ def main() -> int:
     bit_exact = None if gold is None else (digest == gold["forward"])
     del check
 
-    # Dominant kernel: the stage-range pass; log_n stages over `passes` launches
-    # (the library's BN254 Fr plan: one pass up to 2^11, else ceil(log_n / 9)).
-    passes = 1 if log_n <= 11 else -(-log_n // 9)
+    passes = fr_passes(log_n)
     muls_per_transform = (n // 2) * log_n * MULS_PER_MODMUL
     launch_ms = ms_step / passes
     achieved = muls_per_transform / passes / (launch_ms * 1e-3)
-    traffic = None
-    if os.path.exists(TRAFFIC_FILE):
-        try:
-            traffic = json.load(open(TRAFFIC_FILE)).get("hbm_bytes_per_launch")
-        except (ValueError, OSError):
-            traffic = None
     roofline = {"bound": "valu", "achieved": achieved / 1e12, "peak": VALU_MUL_PEAK / 1e12,
-                "unit": "Tmul32/s", "frac": achieved / VALU_MUL_PEAK, "traffic": traffic}
-
+                "unit": "Tmul32/s", "frac": achieved / VALU_MUL_PEAK, "traffic": read_traffic(TRAFFIC_FILE)}
     extras = {"passes_per_transform": passes, "avg_pass_ms": launch_ms, "bit_exact_vs_oracle_digest": bit_exact,
               "input_gen_s": round(gen_s, 2)}
     mad, copy = ctx.probe_peaks()
     extras["measured_mad_peak_Tmul_s"] = mad / 1e12
     extras["measured_copy_GB_s"] = copy / 1e9
     extras["frac_of_measured_mad_peak"] = achieved / mad
-
     if not args.skip_host:
         # PCIe-inclusive rate through the synchronous host-buffer entry point.
         h = x.copy()
@@ -120,10 +121,8 @@ def main() -> int:
         t1 = time.time()
         ctx.ntt(field, h, log_n, root)
         extras["host_buffer_melem_s"] = n / (time.time() - t1) / 1e6
-
     cpu = None
     if not args.skip_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as O
         cl = args.cpu_log_n
         cx = O.gen_uniform(O.FIELD_FR, 1, 1 << cl)
@@ -133,26 +132,137 @@ def main() -> int:
         cs = time.time() - t1
         cpu = {"value": (1 << cl) / cs / 1e6, "unit": "Melem/s", "cores": 1, "kind": "port",
                "sample": f"one forward NTT of 2^{cl} BN254 Fr (seed 1), C oracle -O3, 1 thread, {cs:.1f} s"}
+    line = base_line("Melem/s BN254 Fr NTT 2^24", value, "Melem/s", args, ms_step,
+                     {"workload": f"BN254 Fr forward NTT, 2^{log_n} elements, in place, resident in HBM",
+                      "field": "bn254_fr", "log_n": log_n, "elements": n},
+                     "synthetic: uniform BN254 Fr in [0, r), seed 1 (SplitMix64/xoshiro256++)", "u32")
+    line.update({"roofline": roofline, "cpu_baseline": cpu, "extras": extras})
+    return line
 
-    line = {
-        "metric": "Melem/s BN254 Fr NTT 2^24",
-        "value": value,
-        "unit": "Melem/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "u32",
-        "data": "synthetic: uniform BN254 Fr in [0, r), seed 1 (SplitMix64/xoshiro256++)",
-        "config": {"workload": f"BN254 Fr forward NTT, 2^{log_n} elements, in place, resident in HBM",
-                   "field": "bn254_fr", "log_n": log_n, "elements": n},
-        "roofline": roofline,
-        "cpu_baseline": cpu,
-        "extras": extras,
-    }
+
+def run_gl_batch(args, Z, ctx):
+    import numpy as np
+    import torch
+    field, count, L = Z.FIELD_GL, 4096, 12
+    n = count << L
+    x = Z.fill_uniform(field, 2, n)
+    dev = torch.from_numpy(x.view(np.int64)).cuda()
+    root = Z.primitive_root(field, L)
+    stream = torch.cuda.current_stream()
+    sp = stream.cuda_stream
+    b = Z.Batch(ctx, field, [(Z.FORWARD, L, dev.data_ptr() + i * (8 << L), root) for i in range(count)])
+    tuned_ms = b.autotune(reps=5) if not args.no_tune else None
+    ms = timed(lambda: b.execute(stream=sp), args.steps, args.warmup, stream)
+    b.status(stream=sp)
+    per_call = timed(lambda: [ctx.execute_device(field, Z.FORWARD, L, dev.data_ptr() + i * (8 << L), root, stream=sp)
+                              for i in range(count)], 2, 1, stream)
+    # correctness: one batch execution of the seeded input against the digest
+    dev.copy_(torch.from_numpy(x.view(np.int64)))
+    b.execute(stream=sp)
+    b.status(stream=sp)
+    gold = json.load(open(os.path.join(ROOT, "tests", "golden", "oracle_digests.json")))["goldilocks_batch"]
+    bit_exact = hashlib.sha256(dev.cpu().numpy().tobytes()).hexdigest() == gold["forward"]
+    alg_bytes = 16 * n
+    achieved = alg_bytes / (ms * 1e-3)
+    roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK, "traffic": read_traffic(GL_TRAFFIC_FILE)}
+    cpu = None
+    if not args.skip_cpu:
+        import oracle as O
+        sample = 512  # transforms, ~10 s of one core
+        cx = x[:sample << L].copy()
+        roots = np.zeros(33, dtype=np.uint64)
+        roots[L] = root[0]
+        offs = np.arange(sample, dtype=np.uint64) << np.uint64(L)
+        t1 = time.time()
+        O.ntt_batch_mt(O.FIELD_GL, cx, offs, np.full(sample, L, dtype=np.uint64), roots, 1)
+        cs = time.time() - t1
+        cpu = {"value": (sample << L) / cs / 1e6, "unit": "Melem/s", "cores": 1, "kind": "port",
+               "sample": f"{sample} of the 4096 transforms, C oracle, 1 thread, {cs:.1f} s"}
+    line = base_line("Melem/s Goldilocks batch 4096 x 2^12 NTT", n / (ms * 1e-3) / 1e6, "Melem/s", args, ms,
+                     {"workload": "4096 independent Goldilocks forward NTTs of 2^12, one persistent launch",
+                      "field": "goldilocks", "count": count, "log_n": L, "elements": n},
+                     "synthetic: uniform Goldilocks in [0, p), seed 2", "u64")
+    line.update({"roofline": roofline, "cpu_baseline": cpu,
+                 "extras": {"batch_shape": b.shape(), "autotune_ms": tuned_ms, "per_call_ms": per_call,
+                            "speedup_vs_per_call": per_call / ms, "bit_exact_vs_oracle_digest": bit_exact}})
+    b.close()
+    return line
+
+
+def run_fr_uneven(args, Z, ctx):
+    import numpy as np
+    import torch
+    from zntt_amd import workloads as W
+    field = Z.FIELD_FR
+    spec = W.uneven_batch()
+    shift = Z.from_int(field, 5)
+    stream = torch.cuda.current_stream()
+    sp = stream.cuda_stream
+    bufs, items = [], []
+    for i, (L, d) in enumerate(spec):
+        x = Z.fill_uniform(field, W.item_seed(3, i), 1 << L)
+        t = torch.from_numpy(x.view(np.int64)).cuda()
+        bufs.append(t)
+        items.append((d, L, t.data_ptr(), Z.primitive_root(field, L), shift if d == 2 else None))
+    n_total = sum(1 << L for L, _ in spec)
+    muls = sum(((1 << L) // 2 * L + ((1 << L) if d else 0)) * MULS_PER_MODMUL for L, d in spec)
+    b = Z.Batch(ctx, field, items)
+    tuned_ms = b.autotune(reps=2) if not args.no_tune else None
+    ms = timed(lambda: b.execute(stream=sp), args.steps, args.warmup, stream)
+    b.status(stream=sp)
+
+    def per_call():
+        for (d, L, p, r, s) in items:
+            ctx.execute_device(field, d, L, p, r, s, stream=sp)
+    pc_ms = timed(per_call, max(2, args.steps // 5), 1, stream)
+    achieved = muls / (ms * 1e-3)
+    roofline = {"bound": "valu", "achieved": achieved / 1e12, "peak": VALU_MUL_PEAK / 1e12, "unit": "Tmul32/s",
+                "frac": achieved / VALU_MUL_PEAK, "traffic": None}
+    line = base_line("Melem/s BN254 Fr uneven batch (512 x 2^10..2^20, mixed directions)",
+                     n_total / (ms * 1e-3) / 1e6, "Melem/s", args, ms,
+                     {"workload": "512 BN254 Fr transforms, log2 size uniform 10..20, forward/inverse/"
+                                  "coset-forward, one persistent launch", "field": "bn254_fr",
+                      "count": len(spec), "elements": n_total},
+                     "synthetic: seed 3 metadata, per-transform seeded coefficients", "u32")
+    line.update({"roofline": roofline, "cpu_baseline": None,
+                 "extras": {"batch_shape": b.shape(), "autotune_ms": tuned_ms, "per_call_ms": pc_ms,
+                            "per_call_melem_s": n_total / (pc_ms * 1e-3) / 1e6,
+                            "speedup_vs_per_call": pc_ms / ms}})
+    b.close()
+    return line
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="fr-ntt", choices=["fr-ntt", "gl-batch", "fr-uneven", "serve"])
+    ap.add_argument("--log-n", type=int, default=24)
+    ap.add_argument("--cpu-log-n", type=int, default=24, help="size of the CPU-baseline sample")
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-host", action="store_true", help="skip the PCIe-inclusive host-buffer timing")
+    ap.add_argument("--no-tune", action="store_true", help="batch workloads: keep the default launch shape")
+    ap.add_argument("--rate", type=float, default=2000.0, help="serve: request arrival rate (1/s)")
+    ap.add_argument("--duration", type=float, default=10.0, help="serve: arrival window (s)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(transforms shard with no communication, see DESIGN.md)", file=sys.stderr)
+        return 2
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import zntt_amd as Z
+    ctx = Z.Context(0)
+    if args.workload == "fr-ntt":
+        line = run_fr_ntt(args, Z, ctx)
+    elif args.workload == "gl-batch":
+        line = run_gl_batch(args, Z, ctx)
+    elif args.workload == "fr-uneven":
+        line = run_fr_uneven(args, Z, ctx)
+    else:
+        from zntt_amd import serve
+        line = serve.bench_line(args, Z, ctx)
     print(json.dumps(line), flush=True)
     ctx.close()
     return 0

@@ -181,6 +181,34 @@ typedef struct {
 
 int zntt_execute_device(zntt_ctx *ctx, const zntt_transform *t, void *stream);
 
+/* ------------------------------------------------------------------------ */
+/* Batched transforms.  No reference counterpart.  All transforms of a batch */
+/* share one field; each result equals zntt_execute_device on it alone.      */
+/* ------------------------------------------------------------------------ */
+typedef struct zntt_batch zntt_batch;
+
+/* Plans a batch of device transforms (sizes and directions may differ):
+ * resolves and pins their twiddle / coset tables in the device cache,
+ * allocates per-transform scratch and the work list.  `ts[i].data` stay the
+ * transforms' buffers for the life of the batch.  A shape chosen earlier by
+ * zntt_batch_autotune for the same sizes and directions is reused. */
+int zntt_batch_create(zntt_ctx *ctx, const zntt_transform *ts, size_t count, zntt_batch **out);
+/* Enqueues every transform of the batch on `stream` as one persistent kernel
+ * launch per pass level (at most four), replayed from a HIP graph captured on
+ * the first call.  Do not run two executions of one batch at once (they share
+ * the scratch and work counters). */
+int zntt_batch_execute(zntt_batch *b, void *stream);
+/* Waits for `stream` and reports a HIP error raised by the batch's launches. */
+int zntt_batch_status(zntt_batch *b, void *stream);
+/* Times candidate shapes (tile size, pass length, resident workgroups per CU,
+ * item order) on shadow copies of the data, keeps the fastest for this batch
+ * and for later batches of the same signature.  `best_ms`: its time per
+ * execution.  The caller's data is not modified. */
+int zntt_batch_autotune(zntt_batch *b, int reps, double *best_ms);
+/* The batch's current shape. */
+int zntt_batch_shape(const zntt_batch *b, int *tile_log, int *kmax, int *wgs_per_cu, int *order);
+void zntt_batch_destroy(zntt_batch *b);
+
 /* Blocks until every call enqueued on the context's own stream is done. */
 int zntt_ctx_synchronize(zntt_ctx *ctx);
 

@@ -91,3 +91,19 @@ def test_library_generator_matches_oracle(Z):
     assert hashlib.sha256(Z.fill_uniform(Z.FIELD_FR, 0, 1 << 16).tobytes()).hexdigest() == c["input_digest"]
     assert [O.value(O.FIELD_FR, Z.fill_uniform(Z.FIELD_FR, 0, 4)[i]) for i in range(4)] == \
         pyref.uniform_elements(pyref.FR, 0, 4)
+
+
+def test_workload_metadata_generators():
+    """The workloads module's xoshiro256++ equals the oracle's stream; the
+    uneven batch and arrival schedules are deterministic and in range."""
+    from zntt_amd import workloads as W
+    g = W.Xoshiro256pp(3)
+    assert [g.next() for _ in range(50)] == [int(v) for v in O.xoshiro_stream(3, 50)]
+    spec = W.uneven_batch()
+    assert len(spec) == 512 and spec == W.uneven_batch()
+    assert all(10 <= L <= 20 and d in (0, 1, 2) for L, d in spec)
+    assert {d for _, d in spec} == {0, 1, 2} and {L for L, _ in spec} == set(range(10, 21))
+    arr = W.serve_arrivals(duration=1.0)
+    assert 1800 < len(arr) < 2200
+    assert all(12 <= k <= 22 and b in (2, 3) for _, k, b in arr)
+    assert all(t0 < t1 for (t0, _, _), (t1, _, _) in zip(arr, arr[1:]))

@@ -1,0 +1,146 @@
+"""GPU tests of the batched persistent launch (zntt_batch_*).
+
+Each transform of a batch must equal its oracle result bit for bit, across
+graph replays and after the autotuner has changed the launch shape; the full
+benchmark workloads are checked against committed digests (the Goldilocks
+4096 x 2^12 batch) or against one plain device call per transform (the
+uneven BN254 Fr batch), which the single-transform tests pin to the oracle.
+"""
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+@pytest.fixture(scope="module")
+def Z():
+    import zntt_amd
+    return zntt_amd
+
+
+@pytest.fixture(scope="module")
+def W():
+    from zntt_amd import workloads
+    return workloads
+
+
+@pytest.fixture(scope="module")
+def ctx(Z):
+    c = Z.Context(0)
+    yield c
+    c.close()
+
+
+def to_dev(x):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(x).view(np.int64)).cuda()
+
+
+def to_host(t, shape):
+    return t.cpu().numpy().view(np.uint64).reshape(shape)
+
+
+def oracle_apply(field, x, log_n, direction, shift):
+    root = O.primitive_root(field, log_n)[1]
+    y = x.copy()
+    if direction == 0:
+        st = O.ntt(field, y, log_n, root)
+    elif direction == 1:
+        st = O.intt(field, y, log_n, root)
+    else:
+        st = O.coset_ntt(field, y, log_n, root, shift)
+    assert st == O.OK
+    return y
+
+
+def test_uneven_mixed_batch_matches_oracle(Z, W, ctx):
+    import torch
+    F = Z.FIELD_FR
+    spec = W.uneven_batch(count=24, lo=5, hi=14, seed=33)
+    shift = O.from_int(F, 5)
+    xs = [Z.fill_uniform(F, W.item_seed(33, i), 1 << L) for i, (L, d) in enumerate(spec)]
+    ts = [to_dev(x) for x in xs]
+    items = [(d, L, t.data_ptr(), Z.primitive_root(F, L), shift if d == 2 else None)
+             for (L, d), t in zip(spec, ts)]
+    b = Z.Batch(ctx, F, items)
+    expect = [x.copy() for x in xs]
+    for rnd in range(3):
+        if rnd == 2:
+            before = [t.clone() for t in ts]
+            b.autotune(reps=1)
+            torch.cuda.synchronize()
+            for t0, t1 in zip(before, ts):  # tuning runs on shadow buffers
+                assert torch.equal(t0, t1)
+            print("tuned shape", b.shape(), flush=True)
+        b.execute(stream=torch.cuda.current_stream().cuda_stream)
+        b.status(stream=torch.cuda.current_stream().cuda_stream)
+        expect = [oracle_apply(F, e, L, d, shift) for e, (L, d) in zip(expect, spec)]
+        for i, ((L, d), t) in enumerate(zip(spec, ts)):
+            assert np.array_equal(to_host(t, expect[i].shape), expect[i]), (rnd, i, L, d)
+    b.close()
+
+
+def test_goldilocks_equal_batch_matches_oracle(Z, ctx):
+    F = Z.FIELD_GL
+    count, L = 64, 12
+    x = Z.fill_uniform(F, 77, count << L)
+    t = to_dev(x)
+    root = Z.primitive_root(F, L)
+    stride = (1 << L) * 8
+    b = Z.Batch(ctx, F, [(Z.FORWARD, L, t.data_ptr() + i * stride, root) for i in range(count)])
+    b.execute()
+    b.status()
+    got = to_host(t, x.shape).reshape(count, 1 << L)
+    for i in range(count):
+        row = x[i << L:(i + 1) << L].copy()
+        O.ntt(F, row, L, root)
+        assert np.array_equal(got[i], row), i
+    b.close()
+
+
+def test_goldilocks_4096x2p12_against_oracle_digest(Z, ctx):
+    F = Z.FIELD_GL
+    d = json.load(open(os.path.join(GOLD, "oracle_digests.json")))["goldilocks_batch"]
+    count, L = d["count"], d["log_n"]
+    x = Z.fill_uniform(F, d["seed"], count << L)
+    assert hashlib.sha256(x.tobytes()).hexdigest() == d["input"]
+    t = to_dev(x)
+    root = Z.primitive_root(F, L)
+    b = Z.Batch(ctx, F, [(Z.FORWARD, L, t.data_ptr() + i * (8 << L), root) for i in range(count)])
+    b.execute()
+    b.status()
+    assert hashlib.sha256(to_host(t, x.shape).tobytes()).hexdigest() == d["forward"]
+    b.close()
+
+
+def test_uneven_512_batch_equals_per_call(Z, W, ctx):
+    """The full uneven workload (512 transforms, 2^10..2^20, mixed
+    directions): batched results equal one plain device call per transform."""
+    import torch
+    F = Z.FIELD_FR
+    spec = W.uneven_batch()
+    shift = O.from_int(F, 5)
+    sp = torch.cuda.current_stream().cuda_stream
+    batch_t, single_t, items = [], [], []
+    for i, (L, d) in enumerate(spec):
+        x = Z.fill_uniform(F, W.item_seed(3, i), 1 << L)
+        a, s = to_dev(x), to_dev(x)
+        batch_t.append(a)
+        single_t.append(s)
+        root = Z.primitive_root(F, L)
+        items.append((d, L, a.data_ptr(), root, shift if d == 2 else None))
+        ctx.execute_device(F, d, L, s.data_ptr(), root, shift if d == 2 else None, stream=sp)
+    b = Z.Batch(ctx, F, items)
+    b.execute(stream=sp)
+    b.status(stream=sp)
+    for i, (a, s) in enumerate(zip(batch_t, single_t)):
+        assert torch.equal(a, s), (i, spec[i])
+    b.close()

@@ -1,0 +1,223 @@
+// Persistent kernels for a batch of independent transforms of any mix of
+// sizes and directions, replayed from a captured HIP graph.
+//
+// Every transform is planned into passes exactly as a single transform is
+// (plan_passes), and every pass into tiles.  A work item is one
+// (transform, pass, tile).  The batch runs as one persistent launch per pass
+// level: level p holds pass p of every transform that has one, so items of a
+// level are independent, and workgroups that stay resident pull them from an
+// atomic counter until the level is drained.  Passes of one transform meet at
+// the kernel boundary.  (Inside one launch they could not meet cheaply on
+// MI355X: the eight XCDs have private L2 caches, so a device-scope
+// release/acquire between workgroups compiles to an L2 write-back and an L2
+// invalidate per dependency -- measured at 10x the batch's compute time.)
+#include "batch.hpp"
+#include "pass_kernels.hpp"
+
+namespace zntt {
+
+// One round of a pass with runtime position in the pass (the single-transform
+// kernel has these as template parameters).  Threads at or above `nthr` own
+// no set but still take part in the barrier.  The `u` operand is normalised at
+// every stage, so one body serves both stage parities.
+template <class F, int R>
+__device__ __forceinline__ void round_dyn(typename F::E (&x)[8], const PassArgs &a, uint32_t *lds,
+                                          const TileCtx &c, int l0, bool first_round, bool last_round, int nthr) {
+  using E = typename F::E;
+  constexpr int SET = 1 << R, NSETS = 8 >> R;
+  const bool active = (int)threadIdx.x < nthr;
+  int g[NSETS], base[NSETS];
+  if (active) {
+    sfor<NSETS>([&](auto i) {
+      const int slot = threadIdx.x + i * nthr;
+      g[i] = slot & (c.G - 1);
+      const int rest = slot >> c.lg;
+      base[i] = (rest & ((1 << l0) - 1)) | ((rest >> l0) << (l0 + R));
+    });
+    sfor<NSETS>([&](auto i) {
+      sfor<SET>([&](auto k) {
+        const int mid = base[i] | (k << l0);
+        if (first_round) {
+          if (a.first) {
+            const size_t src = ((size_t)brev_bits(mid, c.K) << c.hi_bits) + c.c0 + g[i];
+            x[i * SET + k] = F::load_user(a.src, src);
+          } else {
+            x[i * SET + k] = F::load_dev(a.src, (c.hi << (c.lo_bits + c.K)) + ((size_t)mid << c.lo_bits) + c.lo0 + g[i]);
+          }
+        } else {
+          x[i * SET + k] = F::lds_get(lds, c.tile, lds_swz(mid * c.G + g[i]));
+        }
+      });
+    });
+    if (first_round && a.first && a.coset) {  // coset pre-scale at the source index
+      sfor<NSETS>([&](auto i) {
+        sfor<SET>([&](auto k) {
+          const int mid = base[i] | (k << l0);
+          const size_t src = ((size_t)brev_bits(mid, c.K) << c.hi_bits) + c.c0 + g[i];
+          x[i * SET + k] = F::mul(x[i * SET + k], F::load_dev(a.coset, src));
+        });
+      });
+    }
+  }
+  // A pass that is a single round reads and writes the caller's buffer in
+  // place: every load must land before any store.
+  if (first_round && last_round) __syncthreads();
+  if (!active) return;
+  sfor<R>([&](auto t) {
+    // This stage's twiddles, loaded just before use to bound register pressure.
+    constexpr int T = decltype(t)::value;
+    const int s = a.s0 + l0 + T;
+    const size_t off = ((size_t)1 << (s - 1)) - 1;
+    E tw[NSETS << T];
+    sfor<NSETS>([&](auto i) {
+      const size_t lo = a.first ? 0 : c.lo0 + g[i];
+      const int jbase = base[i] & ((1 << l0) - 1);
+      sfor<(1 << T)>([&](auto kk) {
+        tw[(i << T) + kk] = F::load_dev(a.tw, off + ((size_t)(jbase | (kk << l0)) << c.lo_bits) + lo);
+      });
+    });
+    sfor<NSETS>([&](auto i) {
+      sfor<SET>([&](auto k) {
+        constexpr int KK = decltype(k)::value;
+        if constexpr ((KK & (1 << T)) == 0) {
+          F::template bfly<true>(x[i * SET + KK], x[i * SET + KK + (1 << T)],
+                                 tw[(i << T) + (KK & ((1 << T) - 1))]);
+        }
+      });
+    });
+  });
+  if (last_round && a.last && a.has_scale) {
+    const E scale = F::from_words(a.scale);
+    sfor<8>([&](auto j) { x[j] = F::mul(x[j], scale); });
+  }
+  sfor<NSETS>([&](auto i) {
+    sfor<SET>([&](auto k) {
+      const int mid = base[i] | (k << l0);
+      if (last_round) {
+        size_t pos;
+        if (a.first)
+          pos = ((size_t)brev_bits((uint32_t)(c.c0 + g[i]), c.hi_bits) << c.K) + mid;
+        else
+          pos = (c.hi << (c.lo_bits + c.K)) + ((size_t)mid << c.lo_bits) + c.lo0 + g[i];
+        if (a.last)
+          F::store_user(a.dst, pos, x[i * SET + k]);
+        else
+          F::store_dev(a.dst, pos, x[i * SET + k]);
+      } else {
+        F::lds_put(lds, c.tile, lds_swz(mid * c.G + g[i]), x[i * SET + k]);
+      }
+    });
+  });
+}
+
+template <class F, int BLOCK>
+__global__ __launch_bounds__(BLOCK, 2) void batch_kernel(const BatchXform *xf, const uint2 *items, uint32_t nitems,
+                                                         uint32_t *next) {
+  using E = typename F::E;
+  extern __shared__ uint32_t lds[];
+  __shared__ uint32_t s_item;
+  for (;;) {
+    if (threadIdx.x == 0) s_item = atomicAdd(next, 1u);
+    __syncthreads();
+    const uint32_t it = s_item;
+    if (it >= nitems) return;
+    const uint2 item = items[it];
+    const BatchXform &X = xf[item.x];
+    const int p = (int)(item.y >> 24);
+    const uint32_t tile_id = item.y & 0xFFFFFFu;
+    PassArgs a;
+    a.first = p == 0;
+    a.last = p == X.npass - 1;
+    a.src = a.first ? X.data : X.scratch;
+    a.dst = a.last ? X.data : X.scratch;
+    a.tw = X.tw;
+    a.coset = a.first ? X.coset : nullptr;
+    a.has_scale = a.last ? X.has_scale : 0;
+#pragma unroll
+    for (int k = 0; k < 9; k++) a.scale[k] = X.scale[k];
+    a.L = X.L;
+    a.s0 = X.s0[p];
+    a.K = X.K[p];
+    a.log_g = X.lg[p];
+    TileCtx c;
+    c.K = a.K;
+    c.lg = a.log_g;
+    c.G = 1 << c.lg;
+    c.tile = c.G << c.K;
+    c.lo_bits = a.s0 - 1;
+    c.hi_bits = a.L - c.lo_bits - c.K;
+    c.hi = c.lo0 = c.c0 = 0;
+    if (a.first) {
+      c.c0 = (size_t)tile_id << c.lg;
+    } else {
+      const int lbb = c.lo_bits - c.lg;
+      c.hi = (size_t)tile_id >> lbb;
+      c.lo0 = ((size_t)tile_id & (((size_t)1 << lbb) - 1)) << c.lg;
+    }
+    const int nthr = c.tile >> 3;
+    const int nr = (c.K + 2) / 3;
+    const int r0 = c.K - 3 * (nr - 1);
+    E x[8];
+    int l0 = 0;
+    for (int ri = 0; ri < nr; ri++) {
+      const int r = ri == 0 ? r0 : 3;
+      const bool fr = ri == 0, lr = ri == nr - 1;
+      if (r == 1)
+        round_dyn<F, 1>(x, a, lds, c, l0, fr, lr, nthr);
+      else if (r == 2)
+        round_dyn<F, 2>(x, a, lds, c, l0, fr, lr, nthr);
+      else
+        round_dyn<F, 3>(x, a, lds, c, l0, fr, lr, nthr);
+      l0 += r;
+      if (!lr) __syncthreads();
+    }
+    __syncthreads();  // LDS and s_item are reused by the next item
+  }
+}
+
+__global__ void batch_reset_kernel(uint32_t *counters, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) counters[i] = 0;
+}
+
+hipError_t launch_batch(int field, const BatchXform *xf, const uint2 *items, const uint32_t *level_start,
+                        int levels, uint32_t *counters, int grid, int tile_log, hipStream_t s) {
+  // Counters are cleared by a kernel (not a memset node) so the sequence can
+  // be captured in a graph and replayed.
+  hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(64), 0, s, counters, (uint32_t)levels);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const unsigned threads = 1u << (tile_log - 3);
+  for (int lv = 0; lv < levels; lv++) {
+    const uint2 *it = items + level_start[lv];
+    const uint32_t n = level_start[lv + 1] - level_start[lv];
+    if (n == 0) continue;
+    const int g = (int)(n < (uint32_t)grid ? n : (uint32_t)grid);
+    if (field == kFr) {
+      if (tile_log > 11) return hipErrorInvalidValue;
+      const size_t lds = (size_t)9 * 4 << tile_log;
+      hipLaunchKernelGGL((batch_kernel<FrT, 256>), dim3(g), dim3(threads), lds, s, xf, it, n, counters + lv);
+    } else {
+      if (tile_log > 12) return hipErrorInvalidValue;
+      const size_t lds = (size_t)2 * 4 << tile_log;
+      hipLaunchKernelGGL((batch_kernel<GlT, 512>), dim3(g), dim3(threads), lds, s, xf, it, n, counters + lv);
+    }
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+hipError_t configure_batch_lds() {
+  static bool done = false;
+  if (done) return hipSuccess;
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&batch_kernel<FrT, 256>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, 9 * 4 * (1 << 11));
+  if (e != hipSuccess) return e;
+  e = hipFuncSetAttribute(reinterpret_cast<const void *>(&batch_kernel<GlT, 512>),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, 2 * 4 * (1 << 12));
+  if (e == hipSuccess) done = true;
+  return e;
+}
+
+}  // namespace zntt

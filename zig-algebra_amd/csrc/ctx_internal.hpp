@@ -1,0 +1,94 @@
+// Context internals shared by the C ABI translation units (zntt_api.cpp,
+// zntt_batch.cpp): the context, the device twiddle cache, budgeted device
+// memory and host-side field helpers on storage-layout elements.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <array>
+#include <list>
+#include <mutex>
+#include <vector>
+
+#include "host_field.hpp"
+#include "kernels.hpp"
+#include "zntt_gpu.h"
+
+namespace zntt_detail {
+// One element in the caller's storage layout (Fr 8 words, Goldilocks 2).
+struct Elem {
+  uint32_t w[8];
+  bool operator==(const Elem &o) const { return memcmp(w, o.w, sizeof w) == 0; }
+};
+}  // namespace zntt_detail
+
+struct TwiddleEntry {
+  int field = 0;
+  int L = 0;
+  std::vector<zntt_detail::Elem> chain;           // chain[k] = root^(2^k), k = 0..L (powers tables)
+  uint32_t *table = nullptr;         // dev layout, 2^L entries (stages + trailing one)
+  size_t bytes = 0;
+  uint32_t *inv_table = nullptr;     // inverse-with-twiddles table, built on demand
+  size_t inv_bytes = 0;
+  std::vector<const void *> key;     // host stage pointers of a registered table
+  uint64_t last_use = 0;
+  int pins = 0;                      // live batch plans reading this table
+};
+
+struct CosetEntry {
+  int field = 0;
+  zntt_detail::Elem shift;
+  size_t count = 0;
+  uint32_t *table = nullptr;  // dev layout, shift^i for i < count
+  size_t bytes = 0;
+  uint64_t last_use = 0;
+  int pins = 0;
+};
+
+struct zntt_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  uint64_t budget = 0;
+  uint64_t in_use = 0;
+  uint64_t tick = 0;
+  uint64_t protect_from = 0;  // entries used since this tick belong to the running call
+  std::mutex mu;
+  std::list<TwiddleEntry> tables;
+  std::list<CosetEntry> cosets;
+  void *scratch = nullptr;
+  size_t scratch_bytes = 0;
+  void *staging = nullptr;
+  size_t staging_bytes = 0;
+  void *aux = nullptr;  // foreign tables and other per-call temporaries
+  size_t aux_bytes = 0;
+};
+
+namespace zntt_detail {
+
+// Holds the context lock for one API call and shields the cache entries that
+// call touches from eviction by its own later allocations.
+struct CallGuard {
+  std::lock_guard<std::mutex> lock;
+  explicit CallGuard(zntt_ctx *c) : lock(c->mu) { c->protect_from = c->tick + 1; }
+};
+
+bool field_ok(int field);
+size_t user_bytes(int field);
+size_t dev_bytes(int field);
+Elem load_elem(int field, const void *p);
+Elem f_mul(int field, const Elem &a, const Elem &b);
+Elem f_inv(int field, const Elem &a);
+Elem f_from_u64(int field, uint64_t v);
+Elem f_one(int field);
+Elem f_primitive_root(int field, int log_size);
+int two_adicity(int field);
+std::array<uint32_t, 9> f_to_dev(int field, const Elem &a);
+
+int hip_status(hipError_t e);
+void dev_free(zntt_ctx *c, void *p, size_t bytes);
+int dev_alloc(zntt_ctx *c, size_t bytes, void **out);
+int ensure(zntt_ctx *c, void **buf, size_t *have, size_t bytes);
+int power_table(zntt_ctx *c, int field, const Elem &root, int L, hipStream_t s, TwiddleEntry **out);
+int coset_table(zntt_ctx *c, int field, const Elem &shift, size_t count, hipStream_t s, const uint32_t **out);
+
+}  // namespace zntt_detail

@@ -33,8 +33,11 @@ struct PassPlan {
   int tile_log;  // log2 of elements per workgroup tile
 };
 
-// Stage split of a 2^L-point transform into LDS passes.
+// Stage split of a 2^L-point transform into LDS passes (the field's default
+// tile), and the same for an explicit tile of 2^tile_log elements with passes
+// of at most kmax stages (used by the batch autotuner).
 PassPlan plan_passes(int field, int L);
+PassPlan plan_passes_shape(int L, int tile_log, int kmax);
 
 // Enqueue a full transform over `data` (user layout, 2^L elements, in place).
 // `tw` is a flat dev-layout table holding at least 2^L - 1 entries.

@@ -33,8 +33,12 @@ namespace zntt {
 static int tile_log_for(int field) { return field == kFr ? 11 : 12; }
 
 PassPlan plan_passes(int field, int L) {
-  PassPlan p = {};
   const int tl = tile_log_for(field);
+  return plan_passes_shape(L, tl, tl - 2);  // keep >= 4 groups per tile for wide global accesses
+}
+
+PassPlan plan_passes_shape(int L, int tl, int kmax) {
+  PassPlan p = {};
   p.tile_log = tl;
   if (L <= tl) {
     p.passes = 1;
@@ -42,7 +46,6 @@ PassPlan plan_passes(int field, int L) {
     p.log_g[0] = 0;
     return p;
   }
-  const int kmax = tl - 2;  // keep >= 4 groups per tile for wide global accesses
   const int P = (L + kmax - 1) / kmax;
   p.passes = P;
   int s0 = 1;

@@ -14,16 +14,13 @@
 #include "host_field.hpp"
 #include "kernels.hpp"
 #include "zntt_gpu.h"
+#include "ctx_internal.hpp"
 
 using zntt::host::U256;
+using namespace zntt_detail;
 
-namespace {
+namespace zntt_detail {
 
-// One element in the caller's storage layout (Fr 8 words, Goldilocks 2).
-struct Elem {
-  uint32_t w[8];
-  bool operator==(const Elem &o) const { return memcmp(w, o.w, sizeof w) == 0; }
-};
 
 bool field_ok(int field) { return field == ZNTT_FIELD_BN254_FR || field == ZNTT_FIELD_GOLDILOCKS; }
 size_t user_bytes(int field) { return field == ZNTT_FIELD_BN254_FR ? 32 : 8; }
@@ -84,58 +81,14 @@ std::array<uint32_t, 9> f_to_dev(int field, const Elem &a) {
   return d;
 }
 
-}  // namespace
+}  // namespace zntt_detail
 
 // ---------------------------------------------------------------------------
 // Context
 // ---------------------------------------------------------------------------
-struct TwiddleEntry {
-  int field = 0;
-  int L = 0;
-  std::vector<Elem> chain;           // chain[k] = root^(2^k), k = 0..L (powers tables)
-  uint32_t *table = nullptr;         // dev layout, 2^L entries (stages + trailing one)
-  size_t bytes = 0;
-  uint32_t *inv_table = nullptr;     // inverse-with-twiddles table, built on demand
-  size_t inv_bytes = 0;
-  std::vector<const void *> key;     // host stage pointers of a registered table
-  uint64_t last_use = 0;
-};
 
-struct CosetEntry {
-  int field = 0;
-  Elem shift;
-  size_t count = 0;
-  uint32_t *table = nullptr;  // dev layout, shift^i for i < count
-  size_t bytes = 0;
-  uint64_t last_use = 0;
-};
+namespace zntt_detail {
 
-struct zntt_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  uint64_t budget = 0;
-  uint64_t in_use = 0;
-  uint64_t tick = 0;
-  uint64_t protect_from = 0;  // entries used since this tick belong to the running call
-  std::mutex mu;
-  std::list<TwiddleEntry> tables;
-  std::list<CosetEntry> cosets;
-  void *scratch = nullptr;
-  size_t scratch_bytes = 0;
-  void *staging = nullptr;
-  size_t staging_bytes = 0;
-  void *aux = nullptr;  // foreign tables and other per-call temporaries
-  size_t aux_bytes = 0;
-};
-
-namespace {
-
-// Holds the context lock for one API call and shields the cache entries that
-// call touches from eviction by its own later allocations.
-struct CallGuard {
-  std::lock_guard<std::mutex> lock;
-  explicit CallGuard(zntt_ctx *c) : lock(c->mu) { c->protect_from = c->tick + 1; }
-};
 
 int hip_status(hipError_t e) {
   if (e == hipSuccess) return ZNTT_OK;
@@ -153,12 +106,13 @@ void dev_free(zntt_ctx *c, void *p, size_t bytes) {
 bool evict_one(zntt_ctx *c) {
   auto tb = c->tables.end();
   for (auto it = c->tables.begin(); it != c->tables.end(); ++it)
-    if (it->key.empty() && it->last_use < c->protect_from &&
+    if (it->key.empty() && it->pins == 0 && it->last_use < c->protect_from &&
         (tb == c->tables.end() || it->last_use < tb->last_use))
       tb = it;
   auto cb = c->cosets.end();
   for (auto it = c->cosets.begin(); it != c->cosets.end(); ++it)
-    if (it->last_use < c->protect_from && (cb == c->cosets.end() || it->last_use < cb->last_use)) cb = it;
+    if (it->pins == 0 && it->last_use < c->protect_from && (cb == c->cosets.end() || it->last_use < cb->last_use))
+      cb = it;
   const bool have_t = tb != c->tables.end(), have_c = cb != c->cosets.end();
   if (!have_t && !have_c) return false;
   hipDeviceSynchronize();  // in-flight work may still read the entry
@@ -444,7 +398,7 @@ int twiddled(zntt_ctx *c, int field, void *data, size_t len, size_t log_n, const
   });
 }
 
-}  // namespace
+}  // namespace zntt_detail
 
 // ---------------------------------------------------------------------------
 // Exported entry points

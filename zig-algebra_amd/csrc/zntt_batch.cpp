@@ -1,0 +1,394 @@
+// Batched transforms: many transforms of uneven size and mixed direction run
+// as one persistent launch per pass level, captured once in a HIP graph and
+// replayed, with the tile shape, pass length, grid occupancy and work order
+// chosen by a small autotuner.  No reference counterpart (zig-ntt transforms one slice per
+// call); each transform's result equals zntt_execute_device on it alone.
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+#include "batch.hpp"
+#include "ctx_internal.hpp"
+
+using namespace zntt_detail;
+
+namespace {
+
+struct Shape {
+  int tile_log;    // elements per workgroup tile = 2^tile_log
+  int kmax;        // longest pass, in stages
+  int wgs_per_cu;  // resident workgroups of the persistent grid per CU
+  int order;       // item order inside a level: 0 transform-major, 1 round-robin over transforms
+  bool operator<(const Shape &o) const {
+    return std::tie(tile_log, kmax, wgs_per_cu, order) < std::tie(o.tile_log, o.kmax, o.wgs_per_cu, o.order);
+  }
+};
+
+Shape default_shape(int field) {
+  return field == ZNTT_FIELD_BN254_FR ? Shape{11, 9, 2, 0} : Shape{12, 10, 3, 0};
+}
+
+}  // namespace
+
+struct zntt_batch {
+  zntt_ctx *ctx = nullptr;
+  int field = 0;
+  std::vector<zntt::BatchXform> xf;       // host copy (data / table pointers filled)
+  std::vector<TwiddleEntry *> pinned_tw;
+  std::vector<CosetEntry *> pinned_cos;
+  void *scratch = nullptr;                 // all multi-pass scratch
+  size_t scratch_bytes = 0;
+  void *dev = nullptr;                     // xf + items + level counters
+  size_t dev_bytes = 0;
+  uint32_t nitems = 0;
+  uint32_t level_start[zntt::kBatchMaxPasses + 1] = {};
+  int levels = 0;
+  zntt::BatchXform *d_xf = nullptr;
+  uint2 *d_items = nullptr;
+  uint32_t *d_counters = nullptr;
+  Shape shape = {};
+  int grid = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  double tuned_ms = 0;
+};
+
+namespace {
+
+void drop_graph(zntt_batch *b) {
+  if (b->exec) hipGraphExecDestroy(b->exec);
+  if (b->graph) hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+}
+
+// (Re)builds the pass plans, the work list and the device copies for `shape`.
+int build_plan(zntt_batch *b, const Shape &shape) {
+  zntt_ctx *c = b->ctx;
+  drop_graph(b);
+  if (b->dev) {
+    hipStreamSynchronize(c->stream);
+    dev_free(c, b->dev, b->dev_bytes);
+    b->dev = nullptr;
+  }
+  std::vector<uint2> items;
+  std::vector<std::vector<uint2>> per_level(zntt::kBatchMaxPasses);
+  int levels = 0;
+  for (size_t t = 0; t < b->xf.size(); t++) {
+    zntt::BatchXform &x = b->xf[t];
+    const zntt::PassPlan plan = zntt::plan_passes_shape(x.L, shape.tile_log, shape.kmax);
+    if (plan.passes > zntt::kBatchMaxPasses) return ZNTT_UNSUPPORTED;
+    x.npass = plan.passes;
+    levels = std::max(levels, plan.passes);
+    int s0 = 1;
+    for (int p = 0; p < plan.passes; p++) {
+      x.K[p] = plan.K[p];
+      x.lg[p] = plan.log_g[p];
+      x.s0[p] = s0;
+      x.tiles[p] = (uint32_t)(((size_t)1 << x.L) >> (plan.K[p] + plan.log_g[p]));
+      s0 += plan.K[p];
+      if (shape.order == 0)
+        for (uint32_t k = 0; k < x.tiles[p]; k++)
+          per_level[p].push_back(make_uint2((uint32_t)t, ((uint32_t)p << 24) | k));
+    }
+  }
+  if (shape.order == 1) {
+    // Round-robin: tile k of every transform before tile k + 1 of any.
+    for (int p = 0; p < levels; p++) {
+      uint32_t most = 0;
+      for (auto &x : b->xf)
+        if (p < x.npass) most = std::max(most, x.tiles[p]);
+      for (uint32_t k = 0; k < most; k++)
+        for (size_t t = 0; t < b->xf.size(); t++)
+          if (p < b->xf[t].npass && k < b->xf[t].tiles[p])
+            per_level[p].push_back(make_uint2((uint32_t)t, ((uint32_t)p << 24) | k));
+    }
+  }
+  b->levels = levels;
+  b->level_start[0] = 0;
+  for (int p = 0; p < levels; p++) {
+    items.insert(items.end(), per_level[p].begin(), per_level[p].end());
+    b->level_start[p + 1] = (uint32_t)items.size();
+  }
+  b->nitems = (uint32_t)items.size();
+  const size_t xf_bytes = b->xf.size() * sizeof(zntt::BatchXform);
+  const size_t it_bytes = items.size() * sizeof(uint2);
+  const size_t st_bytes = zntt::kBatchMaxPasses * sizeof(uint32_t);
+  auto up = [](size_t v) { return (v + 255) & ~(size_t)255; };
+  b->dev_bytes = up(xf_bytes) + up(it_bytes) + up(st_bytes);
+  int st = dev_alloc(c, b->dev_bytes, &b->dev);
+  if (st) return st;
+  uint8_t *base = (uint8_t *)b->dev;
+  b->d_xf = (zntt::BatchXform *)base;
+  b->d_items = (uint2 *)(base + up(xf_bytes));
+  b->d_counters = (uint32_t *)(base + up(xf_bytes) + up(it_bytes));
+  hipError_t e = hipMemcpyAsync(b->d_xf, b->xf.data(), xf_bytes, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(b->d_items, items.data(), it_bytes, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return hip_status(e);
+  int dev = 0, cus = 0;
+  hipGetDevice(&dev);
+  hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  b->grid = std::max(1, cus * shape.wgs_per_cu);
+  b->shape = shape;
+  return ZNTT_OK;
+}
+
+hipError_t enqueue(zntt_batch *b, hipStream_t s) {
+  const int tl = b->shape.tile_log;
+  return zntt::launch_batch(b->field, b->d_xf, b->d_items, b->level_start, b->levels, b->d_counters, b->grid, tl,
+                            s);
+}
+
+void release(zntt_batch *b) {
+  zntt_ctx *c = b->ctx;
+  hipStreamSynchronize(c->stream);
+  hipDeviceSynchronize();
+  drop_graph(b);
+  if (b->dev) dev_free(c, b->dev, b->dev_bytes);
+  if (b->scratch) dev_free(c, b->scratch, b->scratch_bytes);
+  b->dev = b->scratch = nullptr;
+  for (auto *t : b->pinned_tw) t->pins--;
+  for (auto *e : b->pinned_cos) e->pins--;
+  b->pinned_tw.clear();
+  b->pinned_cos.clear();
+}
+
+// Autotune decisions keyed by the batch's signature (field, sizes, directions),
+// shared by every context of the process.
+std::map<std::vector<int>, Shape> g_tuned;
+std::mutex g_tuned_mu;
+
+std::vector<int> signature(int field, const std::vector<zntt::BatchXform> &xf) {
+  std::vector<int> k{field};
+  for (auto &x : xf) k.push_back(x.L * 4 + (x.has_scale ? 1 : 0) + (x.coset ? 2 : 0));
+  std::sort(k.begin() + 1, k.end());
+  return k;
+}
+
+}  // namespace
+
+extern "C" {
+
+int zntt_batch_create(zntt_ctx *c, const zntt_transform *ts, size_t count, zntt_batch **out) {
+  if (!c || !out || (!ts && count)) return ZNTT_INVALID_ARGUMENT;
+  *out = nullptr;
+  if (count == 0 || count > (1u << 24)) return ZNTT_INVALID_ARGUMENT;
+  const int field = ts[0].field;
+  if (!field_ok(field)) return ZNTT_INVALID_ARGUMENT;
+  for (size_t i = 0; i < count; i++) {
+    const zntt_transform &t = ts[i];
+    if (t.field != field || !t.root || !t.data) return ZNTT_INVALID_ARGUMENT;
+    if (t.direction < ZNTT_FORWARD || t.direction > ZNTT_COSET_FORWARD) return ZNTT_INVALID_ARGUMENT;
+    if (t.direction == ZNTT_COSET_FORWARD && !t.shift) return ZNTT_INVALID_ARGUMENT;
+    if (t.log_n >= 64) return ZNTT_LOG_TOO_LARGE;
+    if (t.log_n == 0 || t.log_n > ZNTT_MAX_LOG_N) return ZNTT_UNSUPPORTED;
+  }
+  CallGuard g(c);
+  hipError_t he = zntt::configure_batch_lds();
+  if (he != hipSuccess) return hip_status(he);
+  auto *b = new zntt_batch();
+  b->ctx = c;
+  b->field = field;
+  b->xf.resize(count);
+  // Largest transforms first, so that nested roots (the standard ones) are
+  // served by the first, largest table.
+  std::vector<size_t> order(count);
+  for (size_t i = 0; i < count; i++) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t x) { return ts[a].log_n > ts[x].log_n; });
+  size_t scratch_elems = 0;
+  int st = ZNTT_OK;
+  for (size_t oi = 0; oi < count && st == ZNTT_OK; oi++) {
+    const size_t i = order[oi];
+    const zntt_transform &t = ts[i];
+    zntt::BatchXform &x = b->xf[i];
+    memset(&x, 0, sizeof x);
+    x.L = (int)t.log_n;
+    x.data = (uint32_t *)t.data;
+    Elem root = load_elem(field, t.root);
+    if (t.direction == ZNTT_INVERSE) {
+      root = f_inv(field, root);
+      const auto sc = f_to_dev(field, f_inv(field, f_from_u64(field, (uint64_t)1 << t.log_n)));
+      memcpy(x.scale, sc.data(), sizeof x.scale);
+      x.has_scale = 1;
+    }
+    TwiddleEntry *te = nullptr;
+    st = power_table(c, field, root, x.L, c->stream, &te);
+    if (st) break;
+    te->pins++;
+    b->pinned_tw.push_back(te);
+    x.tw = te->table;
+    if (t.direction == ZNTT_COSET_FORWARD) {
+      const uint32_t *cos = nullptr;
+      st = coset_table(c, field, load_elem(field, t.shift), (size_t)1 << t.log_n, c->stream, &cos);
+      if (st) break;
+      for (auto &ce : c->cosets)
+        if (ce.table == cos) {
+          ce.pins++;
+          b->pinned_cos.push_back(&ce);
+          break;
+        }
+      x.coset = cos;
+    }
+    scratch_elems += (size_t)1 << t.log_n;
+  }
+  if (st == ZNTT_OK) {
+    // Every transform gets its own scratch (any may need several passes
+    // under some tuned shape).
+    b->scratch_bytes = scratch_elems * dev_bytes(field);
+    st = dev_alloc(c, b->scratch_bytes, &b->scratch);
+  }
+  if (st == ZNTT_OK) {
+    size_t off = 0;
+    for (auto &x : b->xf) {
+      x.scratch = (uint32_t *)((uint8_t *)b->scratch + off * dev_bytes(field));
+      off += (size_t)1 << x.L;
+    }
+    Shape sh = default_shape(field);
+    {
+      std::lock_guard<std::mutex> lk(g_tuned_mu);
+      auto it = g_tuned.find(signature(field, b->xf));
+      if (it != g_tuned.end()) sh = it->second;
+    }
+    st = build_plan(b, sh);
+  }
+  if (st) {
+    release(b);
+    delete b;
+    return st;
+  }
+  *out = b;
+  return ZNTT_OK;
+}
+
+int zntt_batch_execute(zntt_batch *b, void *stream) {
+  if (!b) return ZNTT_INVALID_ARGUMENT;
+  zntt_ctx *c = b->ctx;
+  CallGuard g(c);
+  hipStream_t s = (hipStream_t)stream;
+  if (!b->exec) {
+    // Capture on the context's own stream (the null stream cannot be
+    // captured); the instantiated graph then launches on any stream.
+    hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+      hipError_t le = enqueue(b, c->stream);
+      e = hipStreamEndCapture(c->stream, &b->graph);
+      if (le != hipSuccess) e = le;
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      hipGetLastError();
+      drop_graph(b);
+      return hip_status(e);
+    }
+  }
+  return hip_status(hipGraphLaunch(b->exec, s));
+}
+
+int zntt_batch_status(zntt_batch *b, void *stream) {
+  if (!b) return ZNTT_INVALID_ARGUMENT;
+  return hip_status(hipStreamSynchronize((hipStream_t)stream));
+}
+
+int zntt_batch_autotune(zntt_batch *b, int reps, double *best_ms) {
+  if (!b || reps < 1) return ZNTT_INVALID_ARGUMENT;
+  zntt_ctx *c = b->ctx;
+  std::vector<Shape> cands;
+  const int base_tl = b->field == ZNTT_FIELD_BN254_FR ? 11 : 12;
+  for (int tl : {base_tl - 1, base_tl})
+    for (int kd : {2, 3})
+      for (int w : {1, 2, 3, 4})
+        for (int order : {0, 1}) {
+          // LDS bounds the resident workgroups: 160 KiB per CU.
+          const size_t lds = (size_t)zntt::dev_words(b->field) * 4 << tl;
+          if (lds * w > 160 * 1024) continue;
+          cands.push_back(Shape{tl, tl - kd, w, order});
+        }
+  // Time on shadow copies of the data so the caller's buffers are untouched.
+  std::vector<uint32_t *> real(b->xf.size());
+  size_t total = 0;
+  for (auto &x : b->xf) total += ((size_t)1 << x.L) * user_bytes(b->field);
+  void *shadow = nullptr;
+  int st;
+  {
+    CallGuard g(c);
+    st = dev_alloc(c, total, &shadow);
+    if (st) return st;
+    hipMemsetAsync(shadow, 0, total, c->stream);
+    size_t off = 0;
+    for (size_t i = 0; i < b->xf.size(); i++) {
+      real[i] = b->xf[i].data;
+      b->xf[i].data = (uint32_t *)((uint8_t *)shadow + off);
+      off += ((size_t)1 << b->xf[i].L) * user_bytes(b->field);
+    }
+  }
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  Shape best = b->shape;
+  double best_t = 1e30;
+  for (const Shape &sh : cands) {
+    CallGuard g(c);
+    st = build_plan(b, sh);
+    if (st) break;
+    hipError_t e = enqueue(b, c->stream);  // warm-up
+    hipEventRecord(e0, c->stream);
+    for (int r = 0; r < reps && e == hipSuccess; r++) e = enqueue(b, c->stream);
+    hipEventRecord(e1, c->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    if (e != hipSuccess) {
+      st = hip_status(e);
+      break;
+    }
+    float ms = 0;
+    hipEventElapsedTime(&ms, e0, e1);
+    if (ms / reps < best_t) {
+      best_t = ms / reps;
+      best = sh;
+    }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  CallGuard g(c);
+  for (size_t i = 0; i < b->xf.size(); i++) b->xf[i].data = real[i];
+  hipStreamSynchronize(c->stream);
+  dev_free(c, shadow, total);
+  if (st) {
+    build_plan(b, b->shape);
+    return st;
+  }
+  st = build_plan(b, best);
+  if (st) return st;
+  {
+    std::lock_guard<std::mutex> lk(g_tuned_mu);
+    g_tuned[signature(b->field, b->xf)] = best;
+  }
+  b->tuned_ms = best_t;
+  if (best_ms) *best_ms = best_t;
+  return ZNTT_OK;
+}
+
+int zntt_batch_shape(const zntt_batch *b, int *tile_log, int *kmax, int *wgs_per_cu, int *order) {
+  if (!b) return ZNTT_INVALID_ARGUMENT;
+  if (tile_log) *tile_log = b->shape.tile_log;
+  if (kmax) *kmax = b->shape.kmax;
+  if (wgs_per_cu) *wgs_per_cu = b->shape.wgs_per_cu;
+  if (order) *order = b->shape.order;
+  return ZNTT_OK;
+}
+
+void zntt_batch_destroy(zntt_batch *b) {
+  if (!b) return;
+  {
+    CallGuard g(b->ctx);
+    release(b);
+  }
+  delete b;
+}
+
+}  // extern "C"

@@ -85,6 +85,8 @@ EXPORTED = [
     "zntt_free_twiddles", "zntt_ntt_with_twiddles", "zntt_intt_with_twiddles", "zntt_coset_ntt", "zntt_lde",
     "zntt_primitive_root", "zntt_execute_device", "zntt_ctx_synchronize", "zntt_debug_field_op",
     "zntt_probe_peaks", "zntt_fill_uniform", "zntt_ctx_stream",
+    "zntt_batch_create", "zntt_batch_execute", "zntt_batch_status", "zntt_batch_autotune", "zntt_batch_shape",
+    "zntt_batch_destroy",
 ]
 
 
@@ -134,6 +136,14 @@ def _load():
     L.zntt_debug_field_op.argtypes = [vp, i32, i32, vp, vp, vp, sz]
     L.zntt_probe_peaks.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
     L.zntt_fill_uniform.argtypes = [i32, u64, vp, sz]
+    L.zntt_batch_create.argtypes = [vp, ctypes.POINTER(Transform), sz, ctypes.POINTER(vp)]
+    L.zntt_batch_execute.argtypes = [vp, vp]
+    L.zntt_batch_status.argtypes = [vp, vp]
+    L.zntt_batch_autotune.argtypes = [vp, i32, ctypes.POINTER(ctypes.c_double)]
+    ip = ctypes.POINTER(ctypes.c_int)
+    L.zntt_batch_shape.argtypes = [vp, ip, ip, ip, ip]
+    L.zntt_batch_destroy.argtypes = [vp]
+    L.zntt_batch_destroy.restype = None
     return L
 
 
@@ -170,6 +180,18 @@ def fill_uniform(field: int, seed: int, n: int) -> np.ndarray:
     out = empty(field, n)
     check(lib.zntt_fill_uniform(field, seed, _ptr(out), n))
     return out
+
+
+FR_MODULUS = 0x30644E72E131A029B85045B68181585D2833E84879B9709143E1F593F0000001
+GL_MODULUS = 0xFFFFFFFF00000001
+
+
+def from_int(field: int, value: int) -> np.ndarray:
+    """One element in storage layout: Fr in Montgomery form (value * 2^256 mod r)."""
+    if field == FIELD_FR:
+        m = ((value % FR_MODULUS) << 256) % FR_MODULUS
+        return np.array([(m >> (64 * i)) & ((1 << 64) - 1) for i in range(4)], dtype=np.uint64)
+    return np.array([value % GL_MODULUS], dtype=np.uint64)
 
 
 def primitive_root(field: int, log_size: int) -> np.ndarray:
@@ -299,3 +321,57 @@ class Twiddles:
 
     def __getitem__(self, i):
         return self.stages[i]
+
+
+class Batch:
+    """Many device transforms (uneven sizes, mixed directions) run as one
+    persistent launch, replayed from a HIP graph (zntt_batch_*).
+
+    `transforms` is a list of (direction, log_n, data_ptr, root[, shift]) for
+    one field; data_ptr are device pointers that stay the batch's buffers."""
+
+    def __init__(self, ctx: Context, field: int, transforms):
+        self.ctx = ctx
+        self._keep = []
+        arr = (Transform * len(transforms))()
+        for i, t in enumerate(transforms):
+            direction, log_n, ptr, root = t[:4]
+            shift = t[4] if len(t) > 4 else None
+            r = _elem(field, root)
+            s = None if shift is None else _elem(field, shift)
+            self._keep += [r, s]
+            arr[i] = Transform(field, direction, log_n, ptr, r.ctypes.data, None if s is None else s.ctypes.data)
+        h = ctypes.c_void_p()
+        check(lib.zntt_batch_create(ctx.handle, arr, len(transforms), ctypes.byref(h)))
+        self._h = h
+
+    def execute(self, stream: int | None = None) -> None:
+        if stream is None:
+            stream = lib.zntt_ctx_stream(self.ctx.handle)
+        check(lib.zntt_batch_execute(self._h, stream))
+
+    def status(self, stream: int | None = None) -> None:
+        if stream is None:
+            stream = lib.zntt_ctx_stream(self.ctx.handle)
+        check(lib.zntt_batch_status(self._h, stream))
+
+    def autotune(self, reps: int = 3) -> float:
+        ms = ctypes.c_double()
+        check(lib.zntt_batch_autotune(self._h, reps, ctypes.byref(ms)))
+        return ms.value
+
+    def shape(self) -> dict:
+        v = [ctypes.c_int() for _ in range(4)]
+        check(lib.zntt_batch_shape(self._h, *[ctypes.byref(x) for x in v]))
+        return dict(zip(("tile_log", "kmax", "wgs_per_cu", "order"), [x.value for x in v]))
+
+    def close(self) -> None:
+        if self._h:
+            lib.zntt_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
