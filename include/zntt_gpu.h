@@ -191,7 +191,8 @@ typedef struct zntt_batch zntt_batch;
  * resolves and pins their twiddle / coset tables in the device cache,
  * allocates per-transform scratch and the work list.  `ts[i].data` stay the
  * transforms' buffers for the life of the batch.  A shape chosen earlier by
- * zntt_batch_autotune for the same sizes and directions is reused. */
+ * zntt_batch_autotune for the same sizes and directions is reused.  Sizes
+ * run from 2^3 to 2^ZNTT_MAX_LOG_N (ZNTT_UNSUPPORTED otherwise). */
 int zntt_batch_create(zntt_ctx *ctx, const zntt_transform *ts, size_t count, zntt_batch **out);
 /* Enqueues every transform of the batch on `stream` as one persistent kernel
  * launch per pass level (at most four), replayed from a HIP graph captured on
@@ -208,6 +209,122 @@ int zntt_batch_autotune(zntt_batch *b, int reps, double *best_ms);
 /* The batch's current shape. */
 int zntt_batch_shape(const zntt_batch *b, int *tile_log, int *kmax, int *wgs_per_cu, int *order);
 void zntt_batch_destroy(zntt_batch *b);
+
+/* ------------------------------------------------------------------------ */
+/* Serving loop for coset low-degree extensions.  No reference counterpart   */
+/* (the reference's STARK example evaluates its LDE by Horner's rule,        */
+/* examples/stark_prover.zig:117-129).  Requests are queued as they are      */
+/* submitted, admitted while the HBM budget and the active-request limit     */
+/* allow, and run by continuous batching: every scheduler step is one        */
+/* persistent launch holding the next pass of every admitted request, so a   */
+/* request admitted while others are on the device joins the next step.      */
+/* A request's result equals zntt_lde on the same coefficients.              */
+/* ------------------------------------------------------------------------ */
+typedef struct zntt_server zntt_server;
+
+/* Device memory provider for request buffers (inputs the server generates,
+ * outputs, scratch).  `alloc` returns a zntt status; `*out` must be 256-byte
+ * aligned device memory.  Called from the server's scheduler. */
+typedef struct {
+  void *user;
+  int (*alloc)(void *user, size_t bytes, void **out);
+  void (*free)(void *user, void *ptr, size_t bytes);
+} zntt_allocator;
+
+typedef struct {
+  int field;                        /* field of every request                       */
+  uint64_t hbm_budget;              /* request memory limit in bytes (0: 64 GiB)     */
+  int max_active;                   /* admitted requests at once (0: 256)            */
+  int max_log_n;                    /* largest evaluation size 2^max_log_n the server
+                                       prepares twiddles for at creation (0: none)   */
+  int simulate;                     /* 1: no device work, for tests of the scheduling
+                                       on a host without a GPU (ctx may be NULL)     */
+  const zntt_allocator *allocator;  /* NULL: an arena of hbm_budget bytes owned by
+                                       the server (a counter of fake addresses when
+                                       simulating)                                   */
+} zntt_server_config;
+
+typedef struct {
+  size_t log_n;        /* coefficients: 2^log_n                                      */
+  size_t len;          /* number of coefficients; must equal 2^log_n                 */
+  size_t log_blowup;   /* evaluations: 2^(log_n + log_blowup), at least 8            */
+  const void *shift;   /* host pointer, coset shift (one element)                    */
+  const void *coeffs;  /* device pointer to the coefficients, read by the first pass
+                          and kept valid by the caller until the request is done;
+                          NULL: the server generates them into its own memory from
+                          the counter-based stream (seed, stream_id)               */
+  uint64_t seed, stream_id;
+  int keep_result;     /* 1: the evaluations stay in server memory until
+                          zntt_server_release; 0: freed when done (timing only)     */
+  void *out;           /* device pointer for the 2^(log_n+log_blowup) evaluations,
+                          owned by the caller and not charged to the budget; NULL:
+                          server memory (see keep_result)                          */
+} zntt_lde_request;
+
+enum zntt_request_state {
+  ZNTT_REQ_QUEUED = 0,
+  ZNTT_REQ_RUNNING = 1,
+  ZNTT_REQ_DONE = 2,
+  ZNTT_REQ_FAILED = 3,
+  ZNTT_REQ_CANCELLED = 4
+};
+
+typedef struct {
+  int state;           /* zntt_request_state                                */
+  int status;          /* zntt_status of a failed request                   */
+  double queued_ms;    /* submission to admission                           */
+  double latency_ms;   /* submission to completion seen by the scheduler    */
+  void *data;          /* device pointer to the evaluations (kept results)  */
+  size_t log_n, log_blowup;
+} zntt_request_info;
+
+typedef struct {
+  uint64_t submitted, admitted, completed, failed, cancelled;
+  uint64_t steps;           /* scheduler steps launched                      */
+  uint64_t elements_done;   /* evaluations produced                          */
+  uint64_t bytes_in_use, peak_bytes;
+  uint64_t queued, active;  /* requests now waiting / admitted               */
+  double busy_ms;           /* device time of completed steps                */
+} zntt_server_stats;
+
+/* Starts a server on the context's device (a scheduler thread unless
+ * simulating).  Twiddles for sizes up to 2^max_log_n and the coset table of
+ * the field's generator are built up front and stay resident. */
+int zntt_server_create(zntt_ctx *ctx, const zntt_server_config *cfg, zntt_server **out);
+/* Queues a request.  Argument errors are returned here, in the order of
+ * zntt_lde: LogTooLarge, LengthMismatch (len != 2^log_n), OrderTooLarge, and
+ * ZNTT_UNSUPPORTED beyond ZNTT_MAX_LOG_N or below 2^3 evaluations.  A
+ * request larger than the whole budget fails with OutOfMemory when its turn
+ * for admission comes. */
+int zntt_server_submit(zntt_server *s, const zntt_lde_request *r, uint64_t *ticket);
+int zntt_server_query(zntt_server *s, uint64_t ticket, zntt_request_info *out);
+/* Blocks until the request is done, failed or cancelled (timeout_ms < 0:
+ * no limit); ZNTT_DEVICE_ERROR on timeout. */
+int zntt_server_wait(zntt_server *s, uint64_t ticket, double timeout_ms);
+/* A queued request is dropped at once; an admitted one runs to the end of
+ * its passes and its memory is freed then.  Either ends CANCELLED. */
+int zntt_server_cancel(zntt_server *s, uint64_t ticket);
+/* Copies a finished request's evaluations (kept or caller-owned) to host
+ * memory: `len` elements, which must be 2^(log_n+log_blowup)
+ * (ZNTT_LENGTH_MISMATCH otherwise).  Do not release the request meanwhile. */
+int zntt_server_read(zntt_server *s, uint64_t ticket, void *host, size_t len);
+/* Forgets a finished request and frees a kept result. */
+int zntt_server_release(zntt_server *s, uint64_t ticket);
+/* Blocks until no request is queued or admitted. */
+int zntt_server_drain(zntt_server *s);
+int zntt_server_stats_get(zntt_server *s, zntt_server_stats *out);
+/* Simulation only: one non-blocking scheduler iteration (retire, admit,
+ * launch one step); the number of steps then in flight. */
+int zntt_server_sim_pump(zntt_server *s, int *in_flight);
+/* Simulation only: marks the oldest `n` in-flight steps as completed. */
+int zntt_server_sim_complete(zntt_server *s, int n);
+/* Drains, stops the scheduler and frees everything, kept results included. */
+void zntt_server_destroy(zntt_server *s);
+
+/* Device fill with the counter-based stream that generated requests use:
+ * element i of (seed, stream_id) for i < n, in the field's storage layout. */
+int zntt_fill_counter(zntt_ctx *ctx, int field, uint64_t seed, uint64_t stream_id, void *data, size_t n,
+                      void *stream);
 
 /* Blocks until every call enqueued on the context's own stream is done. */
 int zntt_ctx_synchronize(zntt_ctx *ctx);

@@ -72,6 +72,7 @@ def lib():
         L.oracle_ntt_with_twiddles.argtypes = [i32, i32, vp, sz, sz, vp, vp, sz]
         L.oracle_lde.argtypes = [i32, vp, sz, sz, vp, vp]
         L.oracle_gen_uniform.argtypes = [i32, u64, vp, sz]
+        L.oracle_fill_counter.argtypes = [i32, u64, u64, vp, sz]
         L.oracle_xoshiro_stream.argtypes = [u64, vp, sz]
         L.oracle_xoshiro_stream.restype = None
         L.oracle_ntt_batch_mt.argtypes = [i32, vp, vp, vp, vp, sz, i32]
@@ -211,6 +212,14 @@ def lde(field: int, coeffs: np.ndarray, log_k: int, log_blowup: int, shift: np.n
 def gen_uniform(field: int, seed: int, n: int) -> np.ndarray:
     out = empty(field, n)
     st = lib().oracle_gen_uniform(field, seed, _ptr(out), n)
+    assert st == OK
+    return out
+
+
+def fill_counter(field: int, seed: int, stream_id: int, n: int) -> np.ndarray:
+    """Counter-based stream of synthetic request coefficients."""
+    out = empty(field, n)
+    st = lib().oracle_fill_counter(field, seed, stream_id, _ptr(out), n)
     assert st == OK
     return out
 

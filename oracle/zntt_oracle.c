@@ -818,6 +818,50 @@ int oracle_gen_uniform(int field, uint64_t seed, void *out, size_t n) {
   return OR_OK;
 }
 
+/* Counter-based stream for synthetic request coefficients (a request's
+ * coefficients are drawn independently of every other request's, so they can
+ * be generated on the device as requests arrive).  Element i of stream
+ * (seed, id) draws words mix64(key + (ctr + 1) * gamma), where mix64 is the
+ * SplitMix64 finaliser, gamma its increment, key = mix64(mix64(seed) ^ id)
+ * and ctr = i << 8 | attempt << 2 | word.  Fr: four little-endian words, the
+ * top two bits cleared, accepted below r (first of up to 64 attempts) and
+ * kept as the Montgomery words; Goldilocks: one word accepted below p. */
+static uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+int oracle_fill_counter(int field, uint64_t seed, uint64_t stream_id, void *out, size_t n) {
+  fr_init();
+  const uint64_t key = mix64(mix64(seed) ^ stream_id), gamma = 0x9E3779B97F4A7C15ull;
+  if (field == FIELD_FR) {
+    fr_t *o = (fr_t *)out;
+    for (size_t i = 0; i < n; i++) {
+      fr_t v;
+      for (uint64_t att = 0; att < 64; att++) {
+        uint64_t ctr = (uint64_t)i << 8 | att << 2;
+        for (int k = 0; k < 4; k++) v.l[k] = mix64(key + (ctr + (uint64_t)k + 1) * gamma);
+        v.l[3] &= 0x3FFFFFFFFFFFFFFFull;
+        if (fr_lt(v.l, FR_MOD)) break;
+      }
+      o[i] = v;
+    }
+    return OR_OK;
+  }
+  if (field != FIELD_GL) return OR_INVALID_ARGUMENT;
+  uint64_t *o = (uint64_t *)out;
+  for (size_t i = 0; i < n; i++) {
+    uint64_t w = 0;
+    for (uint64_t att = 0; att < 64; att++) {
+      w = mix64(key + (((uint64_t)i << 8 | att << 2) + 1) * gamma);
+      if (w < 0xFFFFFFFF00000001ull) break;
+    }
+    o[i] = w;
+  }
+  return OR_OK;
+}
+
 /* Raw xoshiro256++ stream, for pinning the generator itself. */
 void oracle_xoshiro_stream(uint64_t seed, uint64_t *out, size_t n) {
   xo_t x;

@@ -17,6 +17,8 @@ the mathematics it names, with Python's arbitrary-precision integers.
 * ``horner_coset`` evaluates f(shift * w^k) directly, pinning the coset/LDE
   definition.
 * ``Xoshiro256pp`` is the published xoshiro256++ generator seeded by SplitMix64.
+* ``counter_stored`` restates the counter-based stream of synthetic request
+  coefficients from its definition (SplitMix64's finaliser over a counter).
 """
 from __future__ import annotations
 
@@ -164,3 +166,33 @@ def storage_bytes(p: int, canon: list[int]) -> bytes:
 
 def digest(p: int, canon: list[int]) -> str:
     return hashlib.sha256(storage_bytes(p, canon)).hexdigest()
+
+
+def _mix64(z: int) -> int:
+    """SplitMix64's output finaliser."""
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & MASK64
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & MASK64
+    return z ^ (z >> 31)
+
+
+def counter_stored(p: int, seed: int, stream_id: int, n: int) -> list[int]:
+    """Stored integers of elements 0..n-1 of stream (seed, stream_id): for Fr
+    the 256-bit Montgomery word (four draws, top two bits cleared, first below
+    p), for Goldilocks the canonical value (one draw, first below p).  Draw
+    (i, attempt, word) is mix64(key + (i*256 + attempt*4 + word + 1) * gamma)
+    with key = mix64(mix64(seed) ^ stream_id)."""
+    gamma = 0x9E3779B97F4A7C15
+    key = _mix64(_mix64(seed) ^ stream_id)
+    words = 4 if p == FR else 1
+    out = []
+    for i in range(n):
+        for att in range(64):
+            v = 0
+            for w in range(words):
+                v |= _mix64((key + (i * 256 + att * 4 + w + 1) * gamma) & MASK64) << (64 * w)
+            if p == FR:
+                v &= (1 << 254) - 1
+            if v < p:
+                break
+        out.append(v)
+    return out

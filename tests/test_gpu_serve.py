@@ -1,0 +1,147 @@
+"""GPU tests of the serving loop (zntt_server_*) and the counter-based
+coefficient stream.
+
+Every served LDE must equal the oracle's LDE of the same coefficients bit for
+bit, whether the request ran alone or in a mix of concurrent requests of
+different sizes, blowups and shifts, under a budget that forces queueing, and
+with caller-provided coefficients and outputs as well as generated ones.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def Z():
+    import zntt_amd
+    return zntt_amd
+
+
+@pytest.fixture(scope="module")
+def ctx(Z):
+    c = Z.Context(0)
+    yield c
+    c.close()
+
+
+def to_host(t, field, n):
+    a = t.cpu().numpy().view(np.uint64)
+    return a.reshape(n, 4) if field == O.FIELD_FR else a.reshape(n)
+
+
+def expected(field, seed, sid, k, b, shift):
+    coeffs = O.fill_counter(field, seed, sid, 1 << k)
+    st, out = O.lde(field, coeffs, k, b, shift)
+    assert st == O.OK
+    return out
+
+
+@pytest.mark.parametrize("field", [O.FIELD_FR, O.FIELD_GL])
+def test_device_counter_stream_matches_oracle(Z, ctx, field):
+    import torch
+    words = 4 if field == O.FIELD_FR else 1
+    for seed, sid, n in ((4, 0, 1000), (4, 12345, 4096), (7, 1 << 33, 777)):
+        t = torch.empty(n * words, dtype=torch.int64, device="cuda")
+        Z.fill_counter_device(ctx, field, seed, sid, t.data_ptr(), n,
+                              stream=torch.cuda.current_stream().cuda_stream)
+        torch.cuda.synchronize()
+        assert np.array_equal(to_host(t, field, n), O.fill_counter(field, seed, sid, n)), (seed, sid, n)
+
+
+def mixed_requests(field, count, seed):
+    rng = np.random.default_rng(seed)
+    shifts = [O.from_int(field, 5), O.from_int(field, 7), O.from_int(field, 123456789)]
+    out = []
+    for i in range(count):
+        k = int(rng.integers(3, 13))
+        b = int(rng.integers(1, 4))
+        out.append((k, b, int(rng.integers(0, 3)), 1000 + i))
+    return out, shifts
+
+
+@pytest.mark.parametrize("field", [O.FIELD_FR, O.FIELD_GL])
+def test_mixed_requests_equal_alone_and_oracle(Z, ctx, field):
+    """40 requests of 2^3..2^12 coefficients, blowup 2..8, three shifts, all
+    submitted at once (they share steps) and then one at a time: both runs
+    equal the oracle bit for bit."""
+    reqs, shifts = mixed_requests(field, 40, seed=11 + field)
+    srv = Z.Server(ctx, field, hbm_budget=1 << 30, max_log_n=15)
+    mix = [srv.submit(k, b, shifts[si], seed=4, stream_id=sid, keep=True) for k, b, si, sid in reqs]
+    srv.drain()
+    st = srv.stats()
+    assert st["completed"] == 40 and st["failed"] == 0
+    assert st["steps"] < 40 * 2  # requests shared steps
+    alone = []
+    for k, b, si, sid in reqs:
+        t = srv.submit(k, b, shifts[si], seed=4, stream_id=sid, keep=True)
+        srv.wait(t)
+        alone.append(t)
+    for (k, b, si, sid), tm, ta in zip(reqs, mix, alone):
+        want = expected(field, 4, sid, k, b, shifts[si])
+        got_mix, got_alone = srv.read(tm), srv.read(ta)
+        assert np.array_equal(got_mix, want), (k, b, si, sid)
+        assert np.array_equal(got_alone, got_mix), (k, b, si, sid)
+        q = srv.query(tm)
+        assert q["state"] == Z.REQ_DONE and q["latency_ms"] >= q["queued_ms"] >= 0
+        srv.release(tm)
+        srv.release(ta)
+    assert srv.stats()["bytes_in_use"] == 0
+    srv.close()
+
+
+def test_budget_forces_queueing_results_exact(Z, ctx):
+    """A budget that holds two 2^14-evaluation requests at a time."""
+    F = O.FIELD_FR
+    per = (32 + 36) * (1 << 14) + 32 * (1 << 12)
+    srv = Z.Server(ctx, F, hbm_budget=2 * per + 1024, max_log_n=15)
+    sh = O.from_int(F, 5)
+    ts = [srv.submit(12, 2, sh, seed=4, stream_id=i, keep=True) for i in range(8)]
+    got = []
+    for t in ts:
+        srv.wait(t)
+        got.append(srv.read(t))
+        srv.release(t)
+    st = srv.stats()
+    assert st["completed"] == 8 and st["peak_bytes"] <= 2 * per + 1024
+    for i, g in enumerate(got):
+        assert np.array_equal(g, expected(F, 4, i, 12, 2, sh)), i
+    srv.close()
+
+
+def test_caller_buffers_and_larger_request(Z, ctx):
+    """Caller-owned coefficients (2^16, written by the device generator) and
+    caller-owned output (2^19), beside a generated small request."""
+    import torch
+    F = O.FIELD_FR
+    k, b = 16, 3
+    coeffs = torch.empty((1 << k) * 4, dtype=torch.int64, device="cuda")
+    out = torch.zeros((1 << (k + b)) * 4, dtype=torch.int64, device="cuda")
+    Z.fill_counter_device(ctx, F, 9, 1, coeffs.data_ptr(), 1 << k,
+                          stream=torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    sh = O.from_int(F, 5)
+    srv = Z.Server(ctx, F, max_log_n=20)
+    t_big = srv.submit(k, b, sh, coeffs_ptr=coeffs.data_ptr(), out_ptr=out.data_ptr())
+    t_small = srv.submit(8, 2, sh, seed=9, stream_id=2, keep=True)
+    srv.drain()
+    assert srv.query(t_big)["data"] == out.data_ptr()
+    assert srv.stats()["bytes_in_use"] == (32 << 10)  # only the kept small result
+    assert np.array_equal(to_host(out, F, 1 << (k + b)), expected(F, 9, 1, k, b, sh))
+    assert np.array_equal(srv.read(t_small), expected(F, 9, 2, 8, 2, sh))
+    srv.close()
+
+
+def test_server_matches_per_call_lde(Z, ctx):
+    """The served LDE equals the library's per-call zntt_lde (host buffers)."""
+    F = O.FIELD_FR
+    sh = O.from_int(F, 5)
+    srv = Z.Server(ctx, F, max_log_n=18)
+    t = srv.submit(14, 3, sh, seed=4, stream_id=77, keep=True)
+    srv.wait(t)
+    got = srv.read(t)
+    coeffs = O.fill_counter(F, 4, 77, 1 << 14)
+    assert np.array_equal(got, ctx.lde(F, coeffs, 14, 3, sh))
+    srv.close()

@@ -1,5 +1,5 @@
-// Device-side descriptors of the batched persistent launch (batch_kernels.hip)
-// and its host launcher.
+// Device-side descriptors of the batched persistent launches
+// (batch_kernels.hip) and their host launchers.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -10,10 +10,12 @@ constexpr int kBatchMaxPasses = 4;
 
 // One transform of a batch, as the persistent kernel sees it.
 struct BatchXform {
-  uint32_t *data;         // user layout, 2^L elements, transformed in place
+  const uint32_t *src;    // first pass input, user layout; elements at or past src_len read as zero
+  uint32_t *data;         // user layout, 2^L elements: the result (may equal src)
   uint32_t *scratch;      // dev layout, 2^L elements (multi-pass transforms)
   const uint32_t *tw;     // flat twiddle table (dev layout) covering 2^L
-  const uint32_t *coset;  // shift^i for i < 2^L (dev layout) or null
+  const uint32_t *coset;  // shift^i for i < src_len (dev layout) or null
+  uint64_t src_len;
   uint32_t scale[9];      // n^-1 (dev form) for inverse transforms
   int has_scale;
   int L, npass;
@@ -21,14 +23,29 @@ struct BatchXform {
   uint32_t tiles[kBatchMaxPasses];
 };
 
-// Work item: x = transform index, y = pass << 24 | tile.  Items of pass level
-// lv are items[level_start[lv] .. level_start[lv+1]) (level_start is a host
-// array of levels + 1 offsets); counters[lv] is that level's next-item
-// counter (device memory, cleared by the first launch).  Tiles hold
-// 2^tile_log elements; the workgroup has 2^tile_log / 8 threads; each level's
-// grid is min(grid, items in the level).
-hipError_t launch_batch(int field, const BatchXform *xf, const uint2 *items, const uint32_t *level_start,
-                        int levels, uint32_t *counters, int grid, int tile_log, hipStream_t s);
+// One (transform, pass) of a launch.  Its tiles are the launch's work items
+// first .. first + tiles[pass] - 1; entries are listed with `first` strictly
+// increasing from 0.
+struct BatchEntry {
+  uint32_t xf;
+  uint32_t pass;
+  uint32_t first;
+  uint32_t pad;
+};
+
+// One persistent launch over `nitems` tiles described by `ent[0..nent)`.
+// `*counter` must be zero when the launch starts.  Tiles hold 2^tile_log
+// elements; the workgroup has 2^tile_log / 8 threads; the grid is
+// min(grid, nitems) workgroups.
+hipError_t launch_batch_level(int field, const BatchXform *xf, const BatchEntry *ent, uint32_t nent,
+                              uint32_t nitems, uint32_t *counter, int grid, int tile_log, hipStream_t s);
+// counters[0..n) = 0 by a kernel (capturable in a graph, unlike a memset).
+hipError_t launch_clear_counters(uint32_t *counters, uint32_t n, hipStream_t s);
 hipError_t configure_batch_lds();
+
+// out[i] for i < n: element i of the counter-based stream (seed, stream_id),
+// user layout.
+hipError_t launch_fill_counter(int field, uint32_t *out, size_t n, uint64_t seed, uint64_t stream_id,
+                               hipStream_t s);
 
 }  // namespace zntt

@@ -11,6 +11,8 @@
 // MI355X: the eight XCDs have private L2 caches, so a device-scope
 // release/acquire between workgroups compiles to an L2 write-back and an L2
 // invalidate per dependency -- measured at 10x the batch's compute time.)
+#include <algorithm>
+
 #include "batch.hpp"
 #include "pass_kernels.hpp"
 
@@ -40,7 +42,7 @@ __device__ __forceinline__ void round_dyn(typename F::E (&x)[8], const PassArgs 
         if (first_round) {
           if (a.first) {
             const size_t src = ((size_t)brev_bits(mid, c.K) << c.hi_bits) + c.c0 + g[i];
-            x[i * SET + k] = F::load_user(a.src, src);
+            x[i * SET + k] = src < a.src_len ? F::load_user(a.src, src) : F::zero();
           } else {
             x[i * SET + k] = F::load_dev(a.src, (c.hi << (c.lo_bits + c.K)) + ((size_t)mid << c.lo_bits) + c.lo0 + g[i]);
           }
@@ -54,7 +56,7 @@ __device__ __forceinline__ void round_dyn(typename F::E (&x)[8], const PassArgs 
         sfor<SET>([&](auto k) {
           const int mid = base[i] | (k << l0);
           const size_t src = ((size_t)brev_bits(mid, c.K) << c.hi_bits) + c.c0 + g[i];
-          x[i * SET + k] = F::mul(x[i * SET + k], F::load_dev(a.coset, src));
+          if (src < a.src_len) x[i * SET + k] = F::mul(x[i * SET + k], F::load_dev(a.coset, src));
         });
       });
     }
@@ -111,25 +113,45 @@ __device__ __forceinline__ void round_dyn(typename F::E (&x)[8], const PassArgs 
 }
 
 template <class F, int BLOCK>
-__global__ __launch_bounds__(BLOCK, 2) void batch_kernel(const BatchXform *xf, const uint2 *items, uint32_t nitems,
-                                                         uint32_t *next) {
+__global__ __launch_bounds__(BLOCK, 2) void batch_kernel(const BatchXform *xf, const BatchEntry *ent, uint32_t nent,
+                                                         uint32_t nitems, uint32_t *next) {
   using E = typename F::E;
   extern __shared__ uint32_t lds[];
-  __shared__ uint32_t s_item;
+  __shared__ uint32_t s_item, s_pos[2];
   for (;;) {
     if (threadIdx.x == 0) s_item = atomicAdd(next, 1u);
     __syncthreads();
     const uint32_t it = s_item;
     if (it >= nitems) return;
-    const uint2 item = items[it];
-    const BatchXform &X = xf[item.x];
-    const int p = (int)(item.y >> 24);
-    const uint32_t tile_id = item.y & 0xFFFFFFu;
+    // Find the entry holding item `it`: each step has every thread test one
+    // candidate boundary, narrowing [lo, hi) by a factor of the workgroup
+    // size.  Two LDS slots alternate so that a write never races a read of
+    // the step before.
+    const uint32_t nthreads = blockDim.x;
+    uint32_t lo = 0, hi = nent;
+    int ping = 0;
+    while (hi - lo > 1) {
+      const uint32_t stride = (hi - lo + nthreads - 1) / nthreads;
+      const uint32_t t = lo + threadIdx.x * stride;
+      if (t < hi) {
+        const uint32_t nx = t + stride;
+        if (ent[t].first <= it && (nx >= hi || ent[nx].first > it)) s_pos[ping] = t;
+      }
+      __syncthreads();
+      lo = s_pos[ping];
+      hi = min(lo + stride, hi);
+      ping ^= 1;
+    }
+    const BatchEntry e = ent[lo];
+    const BatchXform &X = xf[e.xf];
+    const int p = (int)e.pass;
+    const uint32_t tile_id = it - e.first;
     PassArgs a;
     a.first = p == 0;
     a.last = p == X.npass - 1;
-    a.src = a.first ? X.data : X.scratch;
+    a.src = a.first ? X.src : X.scratch;
     a.dst = a.last ? X.data : X.scratch;
+    a.src_len = X.src_len;
     a.tw = X.tw;
     a.coset = a.first ? X.coset : nullptr;
     a.has_scale = a.last ? X.has_scale : 0;
@@ -171,41 +193,36 @@ __global__ __launch_bounds__(BLOCK, 2) void batch_kernel(const BatchXform *xf, c
       l0 += r;
       if (!lr) __syncthreads();
     }
-    __syncthreads();  // LDS and s_item are reused by the next item
+    __syncthreads();  // LDS, s_item and s_pos are reused by the next item
   }
 }
 
-__global__ void batch_reset_kernel(uint32_t *counters, uint32_t n) {
+__global__ void clear_counters_kernel(uint32_t *counters, uint32_t n) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) counters[i] = 0;
 }
 
-hipError_t launch_batch(int field, const BatchXform *xf, const uint2 *items, const uint32_t *level_start,
-                        int levels, uint32_t *counters, int grid, int tile_log, hipStream_t s) {
-  // Counters are cleared by a kernel (not a memset node) so the sequence can
-  // be captured in a graph and replayed.
-  hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(64), 0, s, counters, (uint32_t)levels);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
+hipError_t launch_clear_counters(uint32_t *counters, uint32_t n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(clear_counters_kernel, dim3((n + 255) / 256), dim3(256), 0, s, counters, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_level(int field, const BatchXform *xf, const BatchEntry *ent, uint32_t nent,
+                              uint32_t nitems, uint32_t *counter, int grid, int tile_log, hipStream_t s) {
+  if (nitems == 0) return hipSuccess;
   const unsigned threads = 1u << (tile_log - 3);
-  for (int lv = 0; lv < levels; lv++) {
-    const uint2 *it = items + level_start[lv];
-    const uint32_t n = level_start[lv + 1] - level_start[lv];
-    if (n == 0) continue;
-    const int g = (int)(n < (uint32_t)grid ? n : (uint32_t)grid);
-    if (field == kFr) {
-      if (tile_log > 11) return hipErrorInvalidValue;
-      const size_t lds = (size_t)9 * 4 << tile_log;
-      hipLaunchKernelGGL((batch_kernel<FrT, 256>), dim3(g), dim3(threads), lds, s, xf, it, n, counters + lv);
-    } else {
-      if (tile_log > 12) return hipErrorInvalidValue;
-      const size_t lds = (size_t)2 * 4 << tile_log;
-      hipLaunchKernelGGL((batch_kernel<GlT, 512>), dim3(g), dim3(threads), lds, s, xf, it, n, counters + lv);
-    }
-    e = hipGetLastError();
-    if (e != hipSuccess) return e;
+  const int g = (int)(nitems < (uint32_t)grid ? nitems : (uint32_t)grid);
+  if (field == kFr) {
+    if (tile_log > 11) return hipErrorInvalidValue;
+    const size_t lds = (size_t)9 * 4 << tile_log;
+    hipLaunchKernelGGL((batch_kernel<FrT, 256>), dim3(g), dim3(threads), lds, s, xf, ent, nent, nitems, counter);
+  } else {
+    if (tile_log > 12) return hipErrorInvalidValue;
+    const size_t lds = (size_t)2 * 4 << tile_log;
+    hipLaunchKernelGGL((batch_kernel<GlT, 512>), dim3(g), dim3(threads), lds, s, xf, ent, nent, nitems, counter);
   }
-  return hipSuccess;
+  return hipGetLastError();
 }
 
 hipError_t configure_batch_lds() {
@@ -218,6 +235,82 @@ hipError_t configure_batch_lds() {
                           hipFuncAttributeMaxDynamicSharedMemorySize, 2 * 4 * (1 << 12));
   if (e == hipSuccess) done = true;
   return e;
+}
+
+// ---------------------------------------------------------------------------
+// Counter-based input stream (synthetic request coefficients).  Element i of
+// stream (seed, id) draws words mix64(key + (ctr + 1) * gamma) with
+// key = mix64(mix64(seed) ^ id) and ctr = i << 8 | attempt << 2 | word, where
+// mix64 is the SplitMix64 finaliser and gamma its increment.  Fr takes four
+// words (little-endian), clears the top two bits and accepts a value below r
+// (the first of up to 64 attempts; the value is stored as the Montgomery
+// words, so the represented element is uniform too); Goldilocks takes one
+// word and accepts it below p.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+
+constexpr uint64_t kGamma = 0x9e3779b97f4a7c15ull;
+
+__global__ void fill_counter_fr_kernel(uint32_t *out, size_t n, uint64_t key) {
+  const uint64_t r[4] = {0x43e1f593f0000001ull, 0x2833e84879b97091ull, 0xb85045b68181585dull,
+                         0x30644e72e131a029ull};
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    uint64_t w[4] = {0, 0, 0, 0};
+    for (uint64_t att = 0; att < 64; att++) {
+      const uint64_t ctr = (uint64_t)i << 8 | att << 2;
+#pragma unroll
+      for (int k = 0; k < 4; k++) w[k] = mix64(key + (ctr + k + 1) * kGamma);
+      w[3] &= 0x3fffffffffffffffull;
+      bool below = false, decided = false;
+#pragma unroll
+      for (int k = 3; k >= 0; k--) {
+        if (!decided && w[k] != r[k]) {
+          below = w[k] < r[k];
+          decided = true;
+        }
+      }
+      if (below) break;
+    }
+    uint4 *p = reinterpret_cast<uint4 *>(out + i * 8);
+    p[0] = make_uint4((uint32_t)w[0], (uint32_t)(w[0] >> 32), (uint32_t)w[1], (uint32_t)(w[1] >> 32));
+    p[1] = make_uint4((uint32_t)w[2], (uint32_t)(w[2] >> 32), (uint32_t)w[3], (uint32_t)(w[3] >> 32));
+  }
+}
+
+__global__ void fill_counter_gl_kernel(uint32_t *out, size_t n, uint64_t key) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    uint64_t w = 0;
+    for (uint64_t att = 0; att < 64; att++) {
+      w = mix64(key + (((uint64_t)i << 8 | att << 2) + 1) * kGamma);
+      if (w < 0xffffffff00000001ull) break;
+    }
+    reinterpret_cast<uint2 *>(out)[i] = make_uint2((uint32_t)w, (uint32_t)(w >> 32));
+  }
+}
+
+uint64_t counter_key(uint64_t seed, uint64_t stream_id) {
+  auto m = [](uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+  };
+  return m(m(seed) ^ stream_id);
+}
+
+hipError_t launch_fill_counter(int field, uint32_t *out, size_t n, uint64_t seed, uint64_t stream_id,
+                               hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  const uint64_t key = counter_key(seed, stream_id);
+  const size_t blocks = std::min<size_t>((n + 255) / 256, 8192);
+  if (field == kFr)
+    hipLaunchKernelGGL(fill_counter_fr_kernel, dim3((unsigned)blocks), dim3(256), 0, s, out, n, key);
+  else
+    hipLaunchKernelGGL(fill_counter_gl_kernel, dim3((unsigned)blocks), dim3(256), 0, s, out, n, key);
+  return hipGetLastError();
 }
 
 }  // namespace zntt

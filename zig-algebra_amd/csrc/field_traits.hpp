@@ -58,6 +58,12 @@ struct FrT {
     for (int k = 0; k < 9; k++) x.v[k] = w[k];
     return x;
   }
+  __device__ static E zero() {
+    E x;
+#pragma unroll
+    for (int k = 0; k < 9; k++) x.v[k] = 0;
+    return x;
+  }
   __device__ static E mul(const E &a, const E &b) { return fr_mul(a, b); }
   template <bool NORM = true>
   __device__ static void bfly(E &u, E &v, const E &w) {
@@ -97,6 +103,7 @@ struct GlT {
     lds[plane + i] = (uint32_t)(x >> 32);
   }
   __device__ static E from_words(const uint32_t *w) { return (uint64_t)w[0] | ((uint64_t)w[1] << 32); }
+  __device__ static E zero() { return 0; }
   __device__ static E mul(const E &a, const E &b) { return gl_mul(a, b); }
   template <bool NORM = true>
   __device__ static void bfly(E &u, E &v, const E &w) {

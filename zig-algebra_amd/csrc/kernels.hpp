@@ -24,6 +24,7 @@ struct PassArgs {
   int has_scale;
   int L, s0, K, log_g;     // tile = 2^log_g groups of 2^K elements
   int first, last;
+  uint64_t src_len;        // batch kernel, first pass: elements at or past it read as zero
 };
 
 struct PassPlan {

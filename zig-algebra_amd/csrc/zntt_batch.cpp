@@ -23,7 +23,7 @@ struct Shape {
   int tile_log;    // elements per workgroup tile = 2^tile_log
   int kmax;        // longest pass, in stages
   int wgs_per_cu;  // resident workgroups of the persistent grid per CU
-  int order;       // item order inside a level: 0 transform-major, 1 round-robin over transforms
+  int order;       // transform order inside a level: 0 as given, 1 largest first
   bool operator<(const Shape &o) const {
     return std::tie(tile_log, kmax, wgs_per_cu, order) < std::tie(o.tile_log, o.kmax, o.wgs_per_cu, o.order);
   }
@@ -43,13 +43,13 @@ struct zntt_batch {
   std::vector<CosetEntry *> pinned_cos;
   void *scratch = nullptr;                 // all multi-pass scratch
   size_t scratch_bytes = 0;
-  void *dev = nullptr;                     // xf + items + level counters
+  void *dev = nullptr;                     // xf + entries + level counters
   size_t dev_bytes = 0;
-  uint32_t nitems = 0;
-  uint32_t level_start[zntt::kBatchMaxPasses + 1] = {};
+  uint32_t level_ent[zntt::kBatchMaxPasses + 1] = {};  // entries of level lv: [level_ent[lv], level_ent[lv+1])
+  uint32_t level_items[zntt::kBatchMaxPasses] = {};
   int levels = 0;
   zntt::BatchXform *d_xf = nullptr;
-  uint2 *d_items = nullptr;
+  zntt::BatchEntry *d_ent = nullptr;
   uint32_t *d_counters = nullptr;
   Shape shape = {};
   int grid = 0;
@@ -76,8 +76,6 @@ int build_plan(zntt_batch *b, const Shape &shape) {
     dev_free(c, b->dev, b->dev_bytes);
     b->dev = nullptr;
   }
-  std::vector<uint2> items;
-  std::vector<std::vector<uint2>> per_level(zntt::kBatchMaxPasses);
   int levels = 0;
   for (size_t t = 0; t < b->xf.size(); t++) {
     zntt::BatchXform &x = b->xf[t];
@@ -92,32 +90,31 @@ int build_plan(zntt_batch *b, const Shape &shape) {
       x.s0[p] = s0;
       x.tiles[p] = (uint32_t)(((size_t)1 << x.L) >> (plan.K[p] + plan.log_g[p]));
       s0 += plan.K[p];
-      if (shape.order == 0)
-        for (uint32_t k = 0; k < x.tiles[p]; k++)
-          per_level[p].push_back(make_uint2((uint32_t)t, ((uint32_t)p << 24) | k));
     }
   }
-  if (shape.order == 1) {
-    // Round-robin: tile k of every transform before tile k + 1 of any.
-    for (int p = 0; p < levels; p++) {
-      uint32_t most = 0;
-      for (auto &x : b->xf)
-        if (p < x.npass) most = std::max(most, x.tiles[p]);
-      for (uint32_t k = 0; k < most; k++)
-        for (size_t t = 0; t < b->xf.size(); t++)
-          if (p < b->xf[t].npass && k < b->xf[t].tiles[p])
-            per_level[p].push_back(make_uint2((uint32_t)t, ((uint32_t)p << 24) | k));
-    }
-  }
+  // Entries level by level; inside a level the transforms in the batch's
+  // order (order 0) or largest first (order 1, so the longest-running tiles
+  // start early and the tail is short).
+  std::vector<uint32_t> idx(b->xf.size());
+  for (size_t t = 0; t < idx.size(); t++) idx[t] = (uint32_t)t;
+  if (shape.order == 1)
+    std::stable_sort(idx.begin(), idx.end(), [&](uint32_t u, uint32_t v) { return b->xf[u].L > b->xf[v].L; });
+  std::vector<zntt::BatchEntry> ent;
   b->levels = levels;
-  b->level_start[0] = 0;
+  b->level_ent[0] = 0;
   for (int p = 0; p < levels; p++) {
-    items.insert(items.end(), per_level[p].begin(), per_level[p].end());
-    b->level_start[p + 1] = (uint32_t)items.size();
+    uint32_t first = 0;
+    for (uint32_t t : idx) {
+      const zntt::BatchXform &x = b->xf[t];
+      if (p >= x.npass) continue;
+      ent.push_back(zntt::BatchEntry{t, (uint32_t)p, first, 0});
+      first += x.tiles[p];
+    }
+    b->level_ent[p + 1] = (uint32_t)ent.size();
+    b->level_items[p] = first;
   }
-  b->nitems = (uint32_t)items.size();
   const size_t xf_bytes = b->xf.size() * sizeof(zntt::BatchXform);
-  const size_t it_bytes = items.size() * sizeof(uint2);
+  const size_t it_bytes = ent.size() * sizeof(zntt::BatchEntry);
   const size_t st_bytes = zntt::kBatchMaxPasses * sizeof(uint32_t);
   auto up = [](size_t v) { return (v + 255) & ~(size_t)255; };
   b->dev_bytes = up(xf_bytes) + up(it_bytes) + up(st_bytes);
@@ -125,10 +122,10 @@ int build_plan(zntt_batch *b, const Shape &shape) {
   if (st) return st;
   uint8_t *base = (uint8_t *)b->dev;
   b->d_xf = (zntt::BatchXform *)base;
-  b->d_items = (uint2 *)(base + up(xf_bytes));
+  b->d_ent = (zntt::BatchEntry *)(base + up(xf_bytes));
   b->d_counters = (uint32_t *)(base + up(xf_bytes) + up(it_bytes));
   hipError_t e = hipMemcpyAsync(b->d_xf, b->xf.data(), xf_bytes, hipMemcpyHostToDevice, c->stream);
-  if (e == hipSuccess) e = hipMemcpyAsync(b->d_items, items.data(), it_bytes, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(b->d_ent, ent.data(), it_bytes, hipMemcpyHostToDevice, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   if (e != hipSuccess) return hip_status(e);
   int dev = 0, cus = 0;
@@ -139,10 +136,14 @@ int build_plan(zntt_batch *b, const Shape &shape) {
   return ZNTT_OK;
 }
 
+// Clears the level counters, then one persistent launch per pass level.
 hipError_t enqueue(zntt_batch *b, hipStream_t s) {
-  const int tl = b->shape.tile_log;
-  return zntt::launch_batch(b->field, b->d_xf, b->d_items, b->level_start, b->levels, b->d_counters, b->grid, tl,
-                            s);
+  hipError_t e = zntt::launch_clear_counters(b->d_counters, (uint32_t)b->levels, s);
+  for (int lv = 0; lv < b->levels && e == hipSuccess; lv++)
+    e = zntt::launch_batch_level(b->field, b->d_xf, b->d_ent + b->level_ent[lv],
+                                 b->level_ent[lv + 1] - b->level_ent[lv], b->level_items[lv], b->d_counters + lv,
+                                 b->grid, b->shape.tile_log, s);
+  return e;
 }
 
 void release(zntt_batch *b) {
@@ -187,7 +188,7 @@ int zntt_batch_create(zntt_ctx *c, const zntt_transform *ts, size_t count, zntt_
     if (t.direction < ZNTT_FORWARD || t.direction > ZNTT_COSET_FORWARD) return ZNTT_INVALID_ARGUMENT;
     if (t.direction == ZNTT_COSET_FORWARD && !t.shift) return ZNTT_INVALID_ARGUMENT;
     if (t.log_n >= 64) return ZNTT_LOG_TOO_LARGE;
-    if (t.log_n == 0 || t.log_n > ZNTT_MAX_LOG_N) return ZNTT_UNSUPPORTED;
+    if (t.log_n < 3 || t.log_n > ZNTT_MAX_LOG_N) return ZNTT_UNSUPPORTED;  // a tile holds >= 8 elements
   }
   CallGuard g(c);
   hipError_t he = zntt::configure_batch_lds();
@@ -210,6 +211,8 @@ int zntt_batch_create(zntt_ctx *c, const zntt_transform *ts, size_t count, zntt_
     memset(&x, 0, sizeof x);
     x.L = (int)t.log_n;
     x.data = (uint32_t *)t.data;
+    x.src = x.data;
+    x.src_len = (uint64_t)1 << x.L;
     Elem root = load_elem(field, t.root);
     if (t.direction == ZNTT_INVERSE) {
       root = f_inv(field, root);
@@ -324,6 +327,7 @@ int zntt_batch_autotune(zntt_batch *b, int reps, double *best_ms) {
     for (size_t i = 0; i < b->xf.size(); i++) {
       real[i] = b->xf[i].data;
       b->xf[i].data = (uint32_t *)((uint8_t *)shadow + off);
+      b->xf[i].src = b->xf[i].data;
       off += ((size_t)1 << b->xf[i].L) * user_bytes(b->field);
     }
   }
@@ -355,7 +359,7 @@ int zntt_batch_autotune(zntt_batch *b, int reps, double *best_ms) {
   hipEventDestroy(e0);
   hipEventDestroy(e1);
   CallGuard g(c);
-  for (size_t i = 0; i < b->xf.size(); i++) b->xf[i].data = real[i];
+  for (size_t i = 0; i < b->xf.size(); i++) b->xf[i].data = (uint32_t *)(b->xf[i].src = real[i]);
   hipStreamSynchronize(c->stream);
   dev_free(c, shadow, total);
   if (st) {

@@ -87,7 +87,43 @@ EXPORTED = [
     "zntt_probe_peaks", "zntt_fill_uniform", "zntt_ctx_stream",
     "zntt_batch_create", "zntt_batch_execute", "zntt_batch_status", "zntt_batch_autotune", "zntt_batch_shape",
     "zntt_batch_destroy",
+    "zntt_server_create", "zntt_server_submit", "zntt_server_query", "zntt_server_wait", "zntt_server_cancel",
+    "zntt_server_read", "zntt_server_release", "zntt_server_drain", "zntt_server_stats_get", "zntt_server_sim_pump",
+    "zntt_server_sim_complete", "zntt_server_destroy", "zntt_fill_counter",
 ]
+
+REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED, REQ_CANCELLED = 0, 1, 2, 3, 4
+
+ALLOC_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p))
+FREE_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t)
+
+
+class Allocator(ctypes.Structure):
+    _fields_ = [("user", ctypes.c_void_p), ("alloc", ALLOC_FN), ("free", FREE_FN)]
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("field", ctypes.c_int), ("hbm_budget", ctypes.c_uint64), ("max_active", ctypes.c_int),
+                ("max_log_n", ctypes.c_int), ("simulate", ctypes.c_int),
+                ("allocator", ctypes.POINTER(Allocator))]
+
+
+class LdeRequest(ctypes.Structure):
+    _fields_ = [("log_n", ctypes.c_size_t), ("len", ctypes.c_size_t), ("log_blowup", ctypes.c_size_t),
+                ("shift", ctypes.c_void_p), ("coeffs", ctypes.c_void_p), ("seed", ctypes.c_uint64),
+                ("stream_id", ctypes.c_uint64), ("keep_result", ctypes.c_int), ("out", ctypes.c_void_p)]
+
+
+class RequestInfo(ctypes.Structure):
+    _fields_ = [("state", ctypes.c_int), ("status", ctypes.c_int), ("queued_ms", ctypes.c_double),
+                ("latency_ms", ctypes.c_double), ("data", ctypes.c_void_p), ("log_n", ctypes.c_size_t),
+                ("log_blowup", ctypes.c_size_t)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in ("submitted", "admitted", "completed", "failed", "cancelled",
+                                               "steps", "elements_done", "bytes_in_use", "peak_bytes", "queued",
+                                               "active")] + [("busy_ms", ctypes.c_double)]
 
 
 class Transform(ctypes.Structure):
@@ -144,6 +180,20 @@ def _load():
     L.zntt_batch_shape.argtypes = [vp, ip, ip, ip, ip]
     L.zntt_batch_destroy.argtypes = [vp]
     L.zntt_batch_destroy.restype = None
+    L.zntt_server_create.argtypes = [vp, ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)]
+    L.zntt_server_submit.argtypes = [vp, ctypes.POINTER(LdeRequest), ctypes.POINTER(u64)]
+    L.zntt_server_query.argtypes = [vp, u64, ctypes.POINTER(RequestInfo)]
+    L.zntt_server_wait.argtypes = [vp, u64, ctypes.c_double]
+    L.zntt_server_cancel.argtypes = [vp, u64]
+    L.zntt_server_read.argtypes = [vp, u64, vp, sz]
+    L.zntt_server_release.argtypes = [vp, u64]
+    L.zntt_server_drain.argtypes = [vp]
+    L.zntt_server_stats_get.argtypes = [vp, ctypes.POINTER(ServerStats)]
+    L.zntt_server_sim_pump.argtypes = [vp, ip]
+    L.zntt_server_sim_complete.argtypes = [vp, i32]
+    L.zntt_server_destroy.argtypes = [vp]
+    L.zntt_server_destroy.restype = None
+    L.zntt_fill_counter.argtypes = [vp, i32, u64, u64, vp, sz, vp]
     return L
 
 
@@ -368,6 +418,121 @@ class Batch:
     def close(self) -> None:
         if self._h:
             lib.zntt_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def fill_counter_device(ctx: "Context | None", field: int, seed: int, stream_id: int, data_ptr: int, n: int,
+                        stream: int | None = None) -> None:
+    """Device fill with the counter-based stream of generated requests."""
+    if stream is None:
+        stream = lib.zntt_ctx_stream(ctx.handle)
+    check(lib.zntt_fill_counter(ctx.handle, field, seed, stream_id, data_ptr, n, stream))
+
+
+class Server:
+    """Serving loop for coset LDEs (zntt_server_*): requests are admitted
+    against an HBM budget and run by continuous batching, one scheduler step
+    (one persistent launch holding the next pass of every admitted request)
+    at a time.
+
+    With simulate=True nothing runs on a device (ctx may be None): the test
+    drives the scheduler with sim_pump() and completes steps with
+    sim_complete(), to check admission and failure handling on a CPU host.
+    `allocator` is an optional object with alloc(nbytes) -> address (0 or an
+    exception for failure) and free(address, nbytes)."""
+
+    def __init__(self, ctx: "Context | None", field: int = FIELD_FR, hbm_budget: int = 0, max_active: int = 0,
+                 max_log_n: int = 0, simulate: bool = False, allocator=None):
+        self.ctx = ctx
+        self.field = field
+        self._alloc_ref = None
+        alloc_p = None
+        if allocator is not None:
+            def c_alloc(_user, nbytes, out):
+                try:
+                    addr = int(allocator.alloc(int(nbytes)) or 0)
+                except MemoryError:
+                    return OutOfMemory.status
+                except Exception:
+                    return DeviceError.status
+                out[0] = addr
+                return 0 if addr else OutOfMemory.status
+
+            def c_free(_user, addr, nbytes):
+                allocator.free(int(addr or 0), int(nbytes))
+
+            a = Allocator(None, ALLOC_FN(c_alloc), FREE_FN(c_free))
+            self._alloc_ref = a  # the callbacks must outlive the server
+            alloc_p = ctypes.pointer(a)
+        cfg = ServerConfig(field, hbm_budget, max_active, max_log_n, 1 if simulate else 0, alloc_p)
+        h = ctypes.c_void_p()
+        check(lib.zntt_server_create(None if ctx is None else ctx.handle, ctypes.byref(cfg), ctypes.byref(h)))
+        self._h = h
+        self._shift_keep = {}
+
+    def submit(self, log_n: int, log_blowup: int, shift, coeffs_ptr: int | None = None, seed: int = 0,
+               stream_id: int = 0, keep: bool = False, length: int | None = None,
+               out_ptr: int | None = None) -> int:
+        sh = _elem(self.field, shift)
+        key = sh.tobytes()
+        sh = self._shift_keep.setdefault(key, sh)
+        n = (1 << log_n) if length is None else length
+        req = LdeRequest(log_n, n, log_blowup, sh.ctypes.data, coeffs_ptr, seed, stream_id, 1 if keep else 0,
+                         out_ptr)
+        t = ctypes.c_uint64()
+        check(lib.zntt_server_submit(self._h, ctypes.byref(req), ctypes.byref(t)))
+        return t.value
+
+    def query(self, ticket: int) -> dict:
+        info = RequestInfo()
+        check(lib.zntt_server_query(self._h, ticket, ctypes.byref(info)))
+        return {"state": info.state, "status": info.status, "queued_ms": info.queued_ms,
+                "latency_ms": info.latency_ms, "data": info.data or 0, "log_n": info.log_n,
+                "log_blowup": info.log_blowup}
+
+    def wait(self, ticket: int, timeout_ms: float = -1.0) -> dict:
+        check(lib.zntt_server_wait(self._h, ticket, timeout_ms))
+        return self.query(ticket)
+
+    def cancel(self, ticket: int) -> None:
+        check(lib.zntt_server_cancel(self._h, ticket))
+
+    def read(self, ticket: int) -> np.ndarray:
+        """A finished request's evaluations (kept or caller-owned), copied to the host."""
+        q = self.query(ticket)
+        n = 1 << (q["log_n"] + q["log_blowup"])
+        out = empty(self.field, n)
+        check(lib.zntt_server_read(self._h, ticket, _ptr(out), n))
+        return out
+
+    def release(self, ticket: int) -> None:
+        check(lib.zntt_server_release(self._h, ticket))
+
+    def drain(self) -> None:
+        check(lib.zntt_server_drain(self._h))
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        check(lib.zntt_server_stats_get(self._h, ctypes.byref(st)))
+        return {n: getattr(st, n) for n, _ in ServerStats._fields_}
+
+    def sim_pump(self) -> int:
+        n = ctypes.c_int()
+        check(lib.zntt_server_sim_pump(self._h, ctypes.byref(n)))
+        return n.value
+
+    def sim_complete(self, n: int = 1) -> None:
+        check(lib.zntt_server_sim_complete(self._h, n))
+
+    def close(self) -> None:
+        if self._h:
+            lib.zntt_server_destroy(self._h)
             self._h = None
 
     def __del__(self):
