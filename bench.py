@@ -186,9 +186,13 @@ def run_gl_batch(args, Z, ctx):
                      {"workload": "4096 independent Goldilocks forward NTTs of 2^12, one persistent launch",
                       "field": "goldilocks", "count": count, "log_n": L, "elements": n},
                      "synthetic: uniform Goldilocks in [0, p), seed 2", "u64")
+    mad, copy = ctx.probe_peaks()
     line.update({"roofline": roofline, "cpu_baseline": cpu,
                  "extras": {"batch_shape": b.shape(), "autotune_ms": tuned_ms, "per_call_ms": per_call,
-                            "speedup_vs_per_call": per_call / ms, "bit_exact_vs_oracle_digest": bit_exact}})
+                            "speedup_vs_per_call": per_call / ms, "bit_exact_vs_oracle_digest": bit_exact,
+                            "butterflies": count * (1 << (L - 1)) * L,
+                            "measured_copy_GB_s": copy / 1e9, "frac_of_measured_copy": achieved / copy,
+                            "measured_mad_peak_Tmul_s": mad / 1e12}})
     b.close()
     return line
 

@@ -34,11 +34,14 @@ struct BatchEntry {
 };
 
 // One persistent launch over `nitems` tiles described by `ent[0..nent)`.
-// `*counter` must be zero when the launch starts.  Tiles hold 2^tile_log
-// elements; the workgroup has 2^tile_log / 8 threads; the grid is
-// min(grid, nitems) workgroups.
+// `uniform_tiles`: the tile count every entry has, when they are all equal
+// (entry i then starts at item i * uniform_tiles), else 0.  `*counter` must
+// be zero when the launch starts.  Tiles hold 2^tile_log elements; the
+// workgroup has 2^tile_log / 8 threads; the grid is min(grid, nitems)
+// workgroups.
 hipError_t launch_batch_level(int field, const BatchXform *xf, const BatchEntry *ent, uint32_t nent,
-                              uint32_t nitems, uint32_t *counter, int grid, int tile_log, hipStream_t s);
+                              uint32_t nitems, uint32_t uniform_tiles, uint32_t *counter, int grid, int tile_log,
+                              hipStream_t s);
 // counters[0..n) = 0 by a kernel (capturable in a graph, unlike a memset).
 hipError_t launch_clear_counters(uint32_t *counters, uint32_t n, hipStream_t s);
 hipError_t configure_batch_lds();

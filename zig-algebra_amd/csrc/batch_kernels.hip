@@ -29,6 +29,15 @@ __device__ __forceinline__ void round_dyn(typename F::E (&x)[8], const PassArgs 
   constexpr int SET = 1 << R, NSETS = 8 >> R;
   const bool active = (int)threadIdx.x < nthr;
   int g[NSETS], base[NSETS];
+  // A transform that fits one tile (first pass, no outer index bits): its
+  // bit-reversed gather would touch a different cache line per lane, so the
+  // tile is read in natural order (coalesced), scaled and zero-padded, and
+  // written to LDS at the bit-reversed slot; the round then reads LDS.
+  const bool staged = first_round && a.first && c.hi_bits == 0;
+  if (staged) {
+    stage_whole_tile<F>(a, lds, c);
+    __syncthreads();
+  }
   if (active) {
     sfor<NSETS>([&](auto i) {
       const int slot = threadIdx.x + i * nthr;
@@ -39,7 +48,7 @@ __device__ __forceinline__ void round_dyn(typename F::E (&x)[8], const PassArgs 
     sfor<NSETS>([&](auto i) {
       sfor<SET>([&](auto k) {
         const int mid = base[i] | (k << l0);
-        if (first_round) {
+        if (first_round && !staged) {
           if (a.first) {
             const size_t src = ((size_t)brev_bits(mid, c.K) << c.hi_bits) + c.c0 + g[i];
             x[i * SET + k] = src < a.src_len ? F::load_user(a.src, src) : F::zero();
@@ -51,19 +60,24 @@ __device__ __forceinline__ void round_dyn(typename F::E (&x)[8], const PassArgs 
         }
       });
     });
-    if (first_round && a.first && a.coset) {  // coset pre-scale at the source index
-      sfor<NSETS>([&](auto i) {
-        sfor<SET>([&](auto k) {
-          const int mid = base[i] | (k << l0);
-          const size_t src = ((size_t)brev_bits(mid, c.K) << c.hi_bits) + c.c0 + g[i];
-          if (src < a.src_len) x[i * SET + k] = F::mul(x[i * SET + k], F::load_dev(a.coset, src));
-        });
+    if (first_round && a.first && a.coset && !staged) {  // coset pre-scale at the source index
+      // Padding (src >= src_len) was loaded as zero; its factor is zero too.
+      auto cos = [&](int e) {
+        const int i = e / SET, k = e % SET;
+        const int mid = base[i] | (k << l0);
+        const size_t src = ((size_t)brev_bits(mid, c.K) << c.hi_bits) + c.c0 + g[i];
+        return src < a.src_len ? F::load_dev(a.coset, src) : F::zero();
+      };
+      sfor<4>([&](auto h) {
+        constexpr int e = 2 * decltype(h)::value;
+        F::mul2(x[e], cos(e), x[e + 1], cos(e + 1));
       });
     }
   }
   // A pass that is a single round reads and writes the caller's buffer in
-  // place: every load must land before any store.
-  if (first_round && last_round) __syncthreads();
+  // place: every load must land before any store.  A staged round reads LDS
+  // slots that other threads' round outputs overwrite.
+  if ((first_round && last_round) || staged) __syncthreads();
   if (!active) return;
   sfor<R>([&](auto t) {
     // This stage's twiddles, loaded just before use to bound register pressure.
@@ -78,19 +92,19 @@ __device__ __forceinline__ void round_dyn(typename F::E (&x)[8], const PassArgs 
         tw[(i << T) + kk] = F::load_dev(a.tw, off + ((size_t)(jbase | (kk << l0)) << c.lo_bits) + lo);
       });
     });
-    sfor<NSETS>([&](auto i) {
-      sfor<SET>([&](auto k) {
-        constexpr int KK = decltype(k)::value;
-        if constexpr ((KK & (1 << T)) == 0) {
-          F::template bfly<true>(x[i * SET + KK], x[i * SET + KK + (1 << T)],
-                                 tw[(i << T) + (KK & ((1 << T) - 1))]);
-        }
-      });
+    using M = StageMap<R, T>;
+    sfor<2>([&](auto h) {
+      constexpr int j0 = 2 * decltype(h)::value, j1 = j0 + 1;
+      F::template bfly2<true>(x[M::u(j0)], x[M::u(j0) + (1 << T)], tw[(M::set(j0) << T) + M::j(j0)],
+                              x[M::u(j1)], x[M::u(j1) + (1 << T)], tw[(M::set(j1) << T) + M::j(j1)]);
     });
   });
   if (last_round && a.last && a.has_scale) {
     const E scale = F::from_words(a.scale);
-    sfor<8>([&](auto j) { x[j] = F::mul(x[j], scale); });
+    sfor<4>([&](auto h) {
+      constexpr int e = 2 * decltype(h)::value;
+      F::mul2(x[e], scale, x[e + 1], scale);
+    });
   }
   sfor<NSETS>([&](auto i) {
     sfor<SET>([&](auto k) {
@@ -112,24 +126,33 @@ __device__ __forceinline__ void round_dyn(typename F::E (&x)[8], const PassArgs 
   });
 }
 
-template <class F, int BLOCK>
-__global__ __launch_bounds__(BLOCK, 2) void batch_kernel(const BatchXform *xf, const BatchEntry *ent, uint32_t nent,
-                                                         uint32_t nitems, uint32_t *next) {
+template <class F, int BLOCK, int WAVES_PER_SIMD>
+__global__ __launch_bounds__(BLOCK, WAVES_PER_SIMD) void batch_kernel(const BatchXform *xf, const BatchEntry *ent,
+                                                                      uint32_t nent, uint32_t nitems,
+                                                                      uint32_t uniform_tiles, uint32_t *next) {
   using E = typename F::E;
   extern __shared__ uint32_t lds[];
   __shared__ uint32_t s_item, s_pos[2];
+  if (threadIdx.x == 0) s_item = atomicAdd(next, 1u);
+  __syncthreads();
+  uint32_t it = s_item;
   for (;;) {
-    if (threadIdx.x == 0) s_item = atomicAdd(next, 1u);
-    __syncthreads();
-    const uint32_t it = s_item;
     if (it >= nitems) return;
-    // Find the entry holding item `it`: each step has every thread test one
-    // candidate boundary, narrowing [lo, hi) by a factor of the workgroup
-    // size.  Two LDS slots alternate so that a write never races a read of
-    // the step before.
+    // Claim the following item now, so that the atomic's round trip overlaps
+    // this item's work; it is published through s_item at the end.
+    uint32_t claimed = 0;
+    if (threadIdx.x == 0) claimed = atomicAdd(next, 1u);
+    // Find the entry holding item `it`.  Equal tile counts: a division.
+    // Otherwise each step has every thread test one candidate boundary,
+    // narrowing [lo, hi) by a factor of the workgroup size; two LDS slots
+    // alternate so that a write never races a read of the step before.
     const uint32_t nthreads = blockDim.x;
     uint32_t lo = 0, hi = nent;
     int ping = 0;
+    if (uniform_tiles) {
+      lo = it / uniform_tiles;
+      hi = lo + 1;
+    }
     while (hi - lo > 1) {
       const uint32_t stride = (hi - lo + nthreads - 1) / nthreads;
       const uint32_t t = lo + threadIdx.x * stride;
@@ -193,7 +216,10 @@ __global__ __launch_bounds__(BLOCK, 2) void batch_kernel(const BatchXform *xf, c
       l0 += r;
       if (!lr) __syncthreads();
     }
-    __syncthreads();  // LDS, s_item and s_pos are reused by the next item
+    __syncthreads();  // LDS and s_pos are reused by the next item
+    if (threadIdx.x == 0) s_item = claimed;
+    __syncthreads();
+    it = s_item;
   }
 }
 
@@ -209,18 +235,22 @@ hipError_t launch_clear_counters(uint32_t *counters, uint32_t n, hipStream_t s) 
 }
 
 hipError_t launch_batch_level(int field, const BatchXform *xf, const BatchEntry *ent, uint32_t nent,
-                              uint32_t nitems, uint32_t *counter, int grid, int tile_log, hipStream_t s) {
+                              uint32_t nitems, uint32_t uniform_tiles, uint32_t *counter, int grid, int tile_log,
+                              hipStream_t s) {
   if (nitems == 0) return hipSuccess;
+  if (uniform_tiles && (uint64_t)uniform_tiles * nent != nitems) return hipErrorInvalidValue;
   const unsigned threads = 1u << (tile_log - 3);
   const int g = (int)(nitems < (uint32_t)grid ? nitems : (uint32_t)grid);
   if (field == kFr) {
     if (tile_log > 11) return hipErrorInvalidValue;
     const size_t lds = (size_t)9 * 4 << tile_log;
-    hipLaunchKernelGGL((batch_kernel<FrT, 256>), dim3(g), dim3(threads), lds, s, xf, ent, nent, nitems, counter);
+    hipLaunchKernelGGL((batch_kernel<FrT, 256, 2>), dim3(g), dim3(threads), lds, s, xf, ent, nent, nitems, uniform_tiles,
+                       counter);
   } else {
     if (tile_log > 12) return hipErrorInvalidValue;
     const size_t lds = (size_t)2 * 4 << tile_log;
-    hipLaunchKernelGGL((batch_kernel<GlT, 512>), dim3(g), dim3(threads), lds, s, xf, ent, nent, nitems, counter);
+    hipLaunchKernelGGL((batch_kernel<GlT, 512, 4>), dim3(g), dim3(threads), lds, s, xf, ent, nent, nitems, uniform_tiles,
+                       counter);
   }
   return hipGetLastError();
 }
@@ -228,10 +258,10 @@ hipError_t launch_batch_level(int field, const BatchXform *xf, const BatchEntry 
 hipError_t configure_batch_lds() {
   static bool done = false;
   if (done) return hipSuccess;
-  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&batch_kernel<FrT, 256>),
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&batch_kernel<FrT, 256, 2>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, 9 * 4 * (1 << 11));
   if (e != hipSuccess) return e;
-  e = hipFuncSetAttribute(reinterpret_cast<const void *>(&batch_kernel<GlT, 512>),
+  e = hipFuncSetAttribute(reinterpret_cast<const void *>(&batch_kernel<GlT, 512, 4>),
                           hipFuncAttributeMaxDynamicSharedMemorySize, 2 * 4 * (1 << 12));
   if (e == hipSuccess) done = true;
   return e;

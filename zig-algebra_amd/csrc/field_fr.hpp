@@ -107,6 +107,65 @@ __device__ __forceinline__ Fr9 fr_mul(const Fr9 &a, const Fr9 &b) {
   return o;
 }
 
+// Two independent Montgomery products, a0*b0 and a1*b1, with their
+// multiply-add chains interleaved instruction by instruction.  A
+// v_mad_u64_u32 that reads the accumulator written by the instruction just
+// before it costs a wait state on gfx950; alternating two chains gives every
+// multiply-add an independent predecessor, so none is needed.  Bounds as for
+// fr_mul.
+__device__ __forceinline__ void fr_mul2(const Fr9 &a0, const Fr9 &b0, const Fr9 &a1, const Fr9 &b1, Fr9 &o0,
+                                        Fr9 &o1) {
+  constexpr uint32_t P[9] = ZNTT_FR_P29;
+  const uint32_t p0 = fr_opaque(P[0]);
+  uint32_t m0[9], m1[9];
+  uint64_t c0 = 0, c1 = 0;
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+#pragma unroll
+    for (int i = 0; i <= k; i++) {
+      c0 += (uint64_t)a0.v[i] * b0.v[k - i];
+      fr_chain(c0);
+      c1 += (uint64_t)a1.v[i] * b1.v[k - i];
+      fr_chain(c1);
+    }
+#pragma unroll
+    for (int i = 0; i < k; i++) {
+      c0 += (uint64_t)m0[i] * P[k - i];
+      fr_chain(c0);
+      c1 += (uint64_t)m1[i] * P[k - i];
+      fr_chain(c1);
+    }
+    m0[k] = ((uint32_t)c0 * kFrN0) & kM29;
+    m1[k] = ((uint32_t)c1 * kFrN0) & kM29;
+    c0 += (uint64_t)m0[k] * p0;
+    fr_chain(c0);
+    c1 += (uint64_t)m1[k] * p0;
+    fr_chain(c1);
+    c0 >>= 29;
+    c1 >>= 29;
+  }
+#pragma unroll
+  for (int k = 9; k < 17; k++) {
+#pragma unroll
+    for (int i = k - 8; i < 9; i++) {
+      c0 += (uint64_t)a0.v[i] * b0.v[k - i];
+      fr_chain(c0);
+      c1 += (uint64_t)a1.v[i] * b1.v[k - i];
+      fr_chain(c1);
+      c0 += (uint64_t)m0[i] * P[k - i];
+      fr_chain(c0);
+      c1 += (uint64_t)m1[i] * P[k - i];
+      fr_chain(c1);
+    }
+    o0.v[k - 9] = (uint32_t)c0 & kM29;
+    o1.v[k - 9] = (uint32_t)c1 & kM29;
+    c0 >>= 29;
+    c1 >>= 29;
+  }
+  o0.v[8] = (uint32_t)c0;
+  o1.v[8] = (uint32_t)c1;
+}
+
 // Carry-propagate limbs 0..7 into 29 bits; the top limb absorbs the rest.
 __device__ __forceinline__ void fr_norm(Fr9 &x) {
 #pragma unroll
@@ -133,16 +192,23 @@ __device__ __forceinline__ void fr_butterfly(Fr9 &u, Fr9 &v, const Fr9 &w) {
   }
 }
 
-// Butterfly with twiddle -w: (u, v) <- (u - w v, u + w v).
-__device__ __forceinline__ void fr_butterfly_neg(Fr9 &u, Fr9 &v, const Fr9 &w) {
+// Two independent butterflies with their products interleaved (fr_mul2).
+template <bool NORM = true>
+__device__ __forceinline__ void fr_butterfly2(Fr9 &u0, Fr9 &v0, const Fr9 &w0, Fr9 &u1, Fr9 &v1, const Fr9 &w1) {
   constexpr uint32_t C[9] = ZNTT_FR_C4R;
-  Fr9 t = fr_mul(v, w);
-  fr_norm(u);
+  Fr9 t0, t1;
+  fr_mul2(v0, w0, v1, w1, t0, t1);
+  if (NORM) {
+    fr_norm(u0);
+    fr_norm(u1);
+  }
 #pragma unroll
   for (int i = 0; i < 9; i++) {
-    uint32_t ui = u.v[i];
-    v.v[i] = ui + t.v[i];
-    u.v[i] = (ui + C[i]) - t.v[i];
+    const uint32_t a = u0.v[i], b = u1.v[i];
+    u0.v[i] = a + t0.v[i];
+    v0.v[i] = (a + C[i]) - t0.v[i];
+    u1.v[i] = b + t1.v[i];
+    v1.v[i] = (b + C[i]) - t1.v[i];
   }
 }
 

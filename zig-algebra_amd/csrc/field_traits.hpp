@@ -65,9 +65,21 @@ struct FrT {
     return x;
   }
   __device__ static E mul(const E &a, const E &b) { return fr_mul(a, b); }
+  // x0 *= b0 and x1 *= b1 with the two products interleaved.
+  __device__ static void mul2(E &x0, const E &b0, E &x1, const E &b1) {
+    E t0, t1;
+    fr_mul2(x0, b0, x1, b1, t0, t1);
+    x0 = t0;
+    x1 = t1;
+  }
   template <bool NORM = true>
   __device__ static void bfly(E &u, E &v, const E &w) {
     fr_butterfly<NORM>(u, v, w);
+  }
+  // Two independent butterflies issued together.
+  template <bool NORM = true>
+  __device__ static void bfly2(E &u0, E &v0, const E &w0, E &u1, E &v1, const E &w1) {
+    fr_butterfly2<NORM>(u0, v0, w0, u1, v1, w1);
   }
   // Canonical dev-form constant (for tables).
   __device__ static E canon(E x) {
@@ -90,11 +102,15 @@ struct GlT {
   __device__ static E load_user(const uint32_t *base, size_t i) {
     return reinterpret_cast<const uint64_t *>(base)[i];
   }
+  // Transform values are weakly reduced (any u64); the caller's buffer gets
+  // the canonical value.
   __device__ static void store_user(uint32_t *base, size_t i, E x) {
-    reinterpret_cast<uint64_t *>(base)[i] = x;
+    reinterpret_cast<uint64_t *>(base)[i] = gl_canon(x);
   }
   __device__ static E load_dev(const uint32_t *base, size_t i) { return load_user(base, i); }
-  __device__ static void store_dev(uint32_t *base, size_t i, const E &x) { store_user(base, i, x); }
+  __device__ static void store_dev(uint32_t *base, size_t i, const E &x) {
+    reinterpret_cast<uint64_t *>(base)[i] = x;
+  }
   __device__ static E lds_get(const uint32_t *lds, int plane, int i) {
     return (uint64_t)lds[i] | ((uint64_t)lds[plane + i] << 32);
   }
@@ -105,9 +121,18 @@ struct GlT {
   __device__ static E from_words(const uint32_t *w) { return (uint64_t)w[0] | ((uint64_t)w[1] << 32); }
   __device__ static E zero() { return 0; }
   __device__ static E mul(const E &a, const E &b) { return gl_mul(a, b); }
+  __device__ static void mul2(E &x0, const E &b0, E &x1, const E &b1) {
+    x0 = gl_mul(x0, b0);
+    x1 = gl_mul(x1, b1);
+  }
   template <bool NORM = true>
   __device__ static void bfly(E &u, E &v, const E &w) {
     gl_butterfly(u, v, w);
+  }
+  template <bool NORM = true>
+  __device__ static void bfly2(E &u0, E &v0, const E &w0, E &u1, E &v1, const E &w1) {
+    gl_butterfly(u0, v0, w0);
+    gl_butterfly(u1, v1, w1);
   }
   __device__ static E canon(E x) { return gl_canon(x); }
   __device__ static E neg(const E &x) { return gl_neg(x); }

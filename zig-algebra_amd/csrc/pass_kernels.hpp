@@ -129,6 +129,38 @@ __device__ __forceinline__ void sfor(Fn &&f) {
 
 // ODD: parity of the pass's first stage s0, so that the normalise-at-odd-stage
 // rule folds to a constant and each round is one straight-line block.
+// Whole-tile staging of a first pass with no outer index bits (the transform
+// fits one tile): element e of the transform (natural order, coalesced) goes
+// to LDS slot brev(e), multiplied by coset[e] when the pass has
+// a coset, zero at and past src_len.  Uses every thread of the workgroup.
+template <class F>
+__device__ __forceinline__ void stage_whole_tile(const PassArgs &a, uint32_t *lds, const TileCtx &c) {
+  const int n = c.tile;
+  for (int e = threadIdx.x; e < n; e += blockDim.x) {
+    typename F::E v = F::zero();
+    if ((uint64_t)e < a.src_len) {
+      v = F::load_user(a.src, e);
+      if (a.coset) v = F::mul(v, F::load_dev(a.coset, e));
+    }
+    F::lds_put(lds, c.tile, lds_swz(brev_bits(e, c.K)), v);
+  }
+}
+
+// The four butterflies of radix-2 stage T inside a round of R stages over a
+// thread's eight elements (8 >> R sets of 2^R): butterfly j couples elements
+// u(j) and u(j) + 2^T of set set(j), with twiddle index j(j) within the stage.
+template <int R, int T>
+struct StageMap {
+  static constexpr int SET = 1 << R, PER_SET = SET / 2;
+  static constexpr int set(int j) { return j / PER_SET; }
+  static constexpr int kk(int j) {
+    const int q = j % PER_SET;
+    return ((q >> T) << (T + 1)) | (q & ((1 << T) - 1));
+  }
+  static constexpr int u(int j) { return set(j) * SET + kk(j); }
+  static constexpr int j(int jj) { return kk(jj) & ((1 << T) - 1); }
+};
+
 template <class F, int R, int l0, bool first_round, bool last_round, bool ODD>
 __device__ __forceinline__ void pass_round(typename F::E (&x)[8], const PassArgs &a, uint32_t *lds,
                                            const TileCtx &c) {
@@ -156,12 +188,23 @@ __device__ __forceinline__ void pass_round(typename F::E (&x)[8], const PassArgs
       });
     });
   });
-  // Load the sets.
+  // Load the sets.  A transform that fits the tile is staged through LDS
+  // first (coalesced reads instead of a bit-reversed gather).
+  bool staged = false;
+  if constexpr (first_round) {
+    staged = a.first && c.hi_bits == 0;
+    if (staged) {
+      stage_whole_tile<F>(a, lds, c);
+      __syncthreads();
+    }
+  }
   sfor<NSETS>([&](auto i) {
     sfor<SET>([&](auto k) {
       const int mid = base[i] | (k << l0);
       if constexpr (first_round) {
-        if (a.first) {
+        if (staged) {
+          x[i * SET + k] = F::lds_get(lds, c.tile, lds_swz(mid * c.G + g[i]));
+        } else if (a.first) {
           const size_t src = ((size_t)brev_bits(mid, c.K) << c.hi_bits) + c.c0 + g[i];
           x[i * SET + k] = F::load_user(a.src, src);
         } else {
@@ -174,34 +217,39 @@ __device__ __forceinline__ void pass_round(typename F::E (&x)[8], const PassArgs
     });
   });
   if constexpr (first_round) {
-    if (a.first && a.coset) {  // coset pre-scale: data[i] *= shift^i at the source index
-      sfor<NSETS>([&](auto i) {
-        sfor<SET>([&](auto k) {
-          const int mid = base[i] | (k << l0);
-          const size_t src = ((size_t)brev_bits(mid, c.K) << c.hi_bits) + c.c0 + g[i];
-          x[i * SET + k] = F::mul(x[i * SET + k], F::load_dev(a.coset, src));
-        });
+    if (staged) __syncthreads();  // every staged slot is read before round outputs overwrite LDS
+    if (a.first && a.coset && !staged) {  // coset pre-scale: data[i] *= shift^i at the source index
+      auto cos = [&](int e) {
+        const int i = e / SET, k = e % SET;
+        const int mid = base[i] | (k << l0);
+        return F::load_dev(a.coset, ((size_t)brev_bits(mid, c.K) << c.hi_bits) + c.c0 + g[i]);
+      };
+      sfor<4>([&](auto h) {
+        constexpr int e = 2 * decltype(h)::value;
+        F::mul2(x[e], cos(e), x[e + 1], cos(e + 1));
       });
     }
   }
-  // R radix-2 stages in registers; carry-normalise u at odd stages.
+  // R radix-2 stages in registers; carry-normalise u at odd stages.  The
+  // four butterflies of a stage go out as two interleaved pairs.
   sfor<R>([&](auto t) {
-    constexpr bool norm = (((ODD ? 1 : 0) + l0 + decltype(t)::value) & 1) != 0;
-    sfor<NSETS>([&](auto i) {
-      sfor<SET>([&](auto k) {
-        if constexpr ((decltype(k)::value & (1 << decltype(t)::value)) == 0) {
-          constexpr int T = decltype(t)::value, KK = decltype(k)::value;
-          F::template bfly<norm>(x[i * SET + KK], x[i * SET + KK + (1 << T)],
-                                 tw[i * (SET - 1) + (1 << T) - 1 + (KK & ((1 << T) - 1))]);
-        }
-      });
+    constexpr int T = decltype(t)::value;
+    constexpr bool norm = (((ODD ? 1 : 0) + l0 + T) & 1) != 0;
+    using M = StageMap<R, T>;
+    sfor<2>([&](auto h) {
+      constexpr int j0 = 2 * decltype(h)::value, j1 = j0 + 1;
+      F::template bfly2<norm>(x[M::u(j0)], x[M::u(j0) + (1 << T)], tw[M::set(j0) * (SET - 1) + (1 << T) - 1 + M::j(j0)],
+                              x[M::u(j1)], x[M::u(j1) + (1 << T)], tw[M::set(j1) * (SET - 1) + (1 << T) - 1 + M::j(j1)]);
     });
   });
   // Store the sets (to the positions they came from, or to HBM).
   if constexpr (last_round) {
     if (a.last && a.has_scale) {
       const E scale = F::from_words(a.scale);
-      sfor<8>([&](auto j) { x[j] = F::mul(x[j], scale); });
+      sfor<4>([&](auto h) {
+        constexpr int e = 2 * decltype(h)::value;
+        F::mul2(x[e], scale, x[e + 1], scale);
+      });
     }
   }
   sfor<NSETS>([&](auto i) {
@@ -310,6 +358,7 @@ static hipError_t run_passes_t(int field, int L, uint32_t *data, uint32_t *scrat
     if (a.has_scale)
       for (int k = 0; k < F::W; k++) a.scale[k] = scale_dev[k];
     a.L = L;
+    a.src_len = (uint64_t)1 << L;
     a.s0 = s0;
     a.K = plan.K[i];
     a.log_g = plan.log_g[i];

@@ -47,6 +47,7 @@ struct zntt_batch {
   size_t dev_bytes = 0;
   uint32_t level_ent[zntt::kBatchMaxPasses + 1] = {};  // entries of level lv: [level_ent[lv], level_ent[lv+1])
   uint32_t level_items[zntt::kBatchMaxPasses] = {};
+  uint32_t level_uniform[zntt::kBatchMaxPasses] = {};  // tiles per entry when all equal, else 0
   int levels = 0;
   zntt::BatchXform *d_xf = nullptr;
   zntt::BatchEntry *d_ent = nullptr;
@@ -103,15 +104,19 @@ int build_plan(zntt_batch *b, const Shape &shape) {
   b->levels = levels;
   b->level_ent[0] = 0;
   for (int p = 0; p < levels; p++) {
-    uint32_t first = 0;
+    uint32_t first = 0, uniform = 0;
+    bool equal = true;
     for (uint32_t t : idx) {
       const zntt::BatchXform &x = b->xf[t];
       if (p >= x.npass) continue;
       ent.push_back(zntt::BatchEntry{t, (uint32_t)p, first, 0});
       first += x.tiles[p];
+      if (uniform == 0) uniform = x.tiles[p];
+      equal = equal && uniform == x.tiles[p];
     }
     b->level_ent[p + 1] = (uint32_t)ent.size();
     b->level_items[p] = first;
+    b->level_uniform[p] = equal ? uniform : 0;
   }
   const size_t xf_bytes = b->xf.size() * sizeof(zntt::BatchXform);
   const size_t it_bytes = ent.size() * sizeof(zntt::BatchEntry);
@@ -141,8 +146,8 @@ hipError_t enqueue(zntt_batch *b, hipStream_t s) {
   hipError_t e = zntt::launch_clear_counters(b->d_counters, (uint32_t)b->levels, s);
   for (int lv = 0; lv < b->levels && e == hipSuccess; lv++)
     e = zntt::launch_batch_level(b->field, b->d_xf, b->d_ent + b->level_ent[lv],
-                                 b->level_ent[lv + 1] - b->level_ent[lv], b->level_items[lv], b->d_counters + lv,
-                                 b->grid, b->shape.tile_log, s);
+                                 b->level_ent[lv + 1] - b->level_ent[lv], b->level_items[lv],
+                                 b->level_uniform[lv], b->d_counters + lv, b->grid, b->shape.tile_log, s);
   return e;
 }
 

@@ -529,7 +529,7 @@ struct HipDevice : Device {
     memset(h, 0, 256);  // the step's item counter starts at zero
     auto *hx = (zntt::BatchXform *)(h + 256);
     auto *he = (zntt::BatchEntry *)(h + 256 + align_up(sizeof(zntt::BatchXform) * n, 256));
-    uint32_t first = 0;
+    uint32_t first = 0, uniform = 0;
     for (size_t i = 0; i < n; i++) {
       const Request *r = items[i].r;
       zntt::BatchXform &x = hx[i];
@@ -551,7 +551,10 @@ struct HipDevice : Device {
         s0 += r->plan.K[p];
       }
       he[i] = zntt::BatchEntry{(uint32_t)i, (uint32_t)items[i].pass, first, 0};
-      first += x.tiles[items[i].pass];
+      const uint32_t tiles = x.tiles[items[i].pass];
+      if (i == 0) uniform = tiles;
+      if (tiles != uniform) uniform = 0;
+      first += tiles;
     }
     const size_t bytes = (uint8_t *)(he + n) - h;
     uint8_t *d = (uint8_t *)sl->dev;
@@ -560,7 +563,7 @@ struct HipDevice : Device {
     if (e == hipSuccess)
       e = zntt::launch_batch_level(field, (const zntt::BatchXform *)(d + 256),
                                    (const zntt::BatchEntry *)(d + ((uint8_t *)he - h)), (uint32_t)n, first,
-                                   (uint32_t *)d, grid, tile_log, stream);
+                                   uniform, (uint32_t *)d, grid, tile_log, stream);
     if (e == hipSuccess) e = hipEventRecord(sl->end, stream);
     if (e != hipSuccess) return -hip_status(e);
     sl->id = next_id++;
