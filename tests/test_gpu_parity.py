@@ -180,6 +180,19 @@ def test_foreign_non_power_tables(Z, ctx, field):
             got = x.copy()
             (ctx.intt_with_twiddles if inverse else ctx.ntt_with_twiddles)(field, got, log_n, stages)
             assert np.array_equal(got, exp), (log_n, inverse)
+    # Entry 0 of every stage is 1 but the rest is arbitrary: the device may
+    # then skip the products by entry 0 of the first stages, and only those.
+    one = O.from_int(field, 1)
+    for log_n in (4, 11, 12, 14):
+        stages = [O.gen_uniform(field, 700 + s, 1 << s) for s in range(log_n)]
+        for st in stages:
+            st[0] = one if field == O.FIELD_FR else one[0]
+        x = O.gen_uniform(field, 600 + log_n, 1 << log_n)
+        exp = x.copy()
+        O.ntt_with_twiddles(field, exp, log_n, stages)
+        got = x.copy()
+        ctx.ntt_with_twiddles(field, got, log_n, stages)
+        assert np.array_equal(got, exp), ("entry0=1", log_n)
 
 
 # ---------------------------------------------------------------------------

@@ -42,6 +42,16 @@ struct BatchEntry {
 hipError_t launch_batch_level(int field, const BatchXform *xf, const BatchEntry *ent, uint32_t nent,
                               uint32_t nitems, uint32_t uniform_tiles, uint32_t *counter, int grid, int tile_log,
                               hipStream_t s);
+// One persistent launch over n transforms that each fit one tile of 2^K
+// elements (K = their log size, 9..11 for BN254 Fr, 9..12 for Goldilocks):
+// transform ent[i].xf is item i.  `*counter` must be zero at launch.
+hipError_t launch_whole_fr(int K, const BatchXform *xf, const BatchEntry *ent, uint32_t n, uint32_t *counter,
+                           hipStream_t s);
+hipError_t launch_whole_gl(int K, const BatchXform *xf, const BatchEntry *ent, uint32_t n, uint32_t *counter,
+                           hipStream_t s);
+inline int whole_max_log(int field) { return field == 1 ? 11 : 12; }
+constexpr int kWholeMinLog = 9;
+
 // counters[0..n) = 0 by a kernel (capturable in a graph, unlike a memset).
 hipError_t launch_clear_counters(uint32_t *counters, uint32_t n, hipStream_t s);
 hipError_t configure_batch_lds();

@@ -175,6 +175,7 @@ __global__ __launch_bounds__(BLOCK, WAVES_PER_SIMD) void batch_kernel(const Batc
     a.src = a.first ? X.src : X.scratch;
     a.dst = a.last ? X.data : X.scratch;
     a.src_len = X.src_len;
+    a.tw_one = 1;  // batches use tables of powers
     a.tw = X.tw;
     a.coset = a.first ? X.coset : nullptr;
     a.has_scale = a.last ? X.has_scale : 0;

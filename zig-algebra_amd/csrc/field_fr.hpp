@@ -212,6 +212,21 @@ __device__ __forceinline__ void fr_butterfly2(Fr9 &u0, Fr9 &v0, const Fr9 &w0, F
   }
 }
 
+// Butterfly with twiddle 1: (u, v) <- (u + v, u - v).  v must meet what
+// fr_mul's output meets (carry-normalised, value < 2r): a freshly loaded or
+// multiplied value.
+template <bool NORM = true>
+__device__ __forceinline__ void fr_butterfly_one(Fr9 &u, Fr9 &v) {
+  constexpr uint32_t C[9] = ZNTT_FR_C4R;
+  if (NORM) fr_norm(u);
+#pragma unroll
+  for (int i = 0; i < 9; i++) {
+    const uint32_t ui = u.v[i], ti = v.v[i];
+    u.v[i] = ui + ti;
+    v.v[i] = (ui + C[i]) - ti;
+  }
+}
+
 // x <- x - r if x >= r (x carry-normalised, limbs < 2^29, value < 2^261).
 __device__ __forceinline__ void fr_csub_r(Fr9 &x) {
   constexpr uint32_t P[9] = ZNTT_FR_P29;

@@ -100,4 +100,15 @@ __device__ __forceinline__ void gl_butterfly(uint64_t &u, uint64_t &v, uint64_t 
   v = d;
 }
 
+// Butterfly with twiddle 1 on weakly reduced values.
+__device__ __forceinline__ void gl_butterfly_one(uint64_t &u, uint64_t &v) {
+  const uint64_t t = gl_canon(v);
+  uint64_t s = u + t;
+  s += (s < t) ? kGlEps : 0;
+  uint64_t d = u - t;
+  d -= (u < t) ? kGlEps : 0;
+  u = s;
+  v = d;
+}
+
 }  // namespace zntt

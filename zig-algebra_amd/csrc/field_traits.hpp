@@ -81,6 +81,13 @@ struct FrT {
   __device__ static void bfly2(E &u0, E &v0, const E &w0, E &u1, E &v1, const E &w1) {
     fr_butterfly2<NORM>(u0, v0, w0, u1, v1, w1);
   }
+  // Twiddle 1.  Only where v is a loaded or multiplied value: the first stage
+  // of a transform (stage T of its first round).
+  template <bool NORM = true>
+  __device__ static void bfly_one(E &u, E &v) {
+    fr_butterfly_one<NORM>(u, v);
+  }
+  static constexpr bool one_ok(int T) { return T == 0; }
   // Canonical dev-form constant (for tables).
   __device__ static E canon(E x) {
     fr_canon(x);
@@ -134,6 +141,11 @@ struct GlT {
     gl_butterfly(u0, v0, w0);
     gl_butterfly(u1, v1, w1);
   }
+  template <bool NORM = true>
+  __device__ static void bfly_one(E &u, E &v) {
+    gl_butterfly_one(u, v);
+  }
+  static constexpr bool one_ok(int) { return true; }
   __device__ static E canon(E x) { return gl_canon(x); }
   __device__ static E neg(const E &x) { return gl_neg(x); }
 };

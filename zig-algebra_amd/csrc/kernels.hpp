@@ -25,6 +25,7 @@ struct PassArgs {
   int L, s0, K, log_g;     // tile = 2^log_g groups of 2^K elements
   int first, last;
   uint64_t src_len;        // batch kernel, first pass: elements at or past it read as zero
+  int tw_one;              // entry 0 of every stage of `tw` is 1 (a table of powers)
 };
 
 struct PassPlan {
@@ -43,14 +44,16 @@ PassPlan plan_passes_shape(int L, int tile_log, int kmax);
 // Enqueue a full transform over `data` (user layout, 2^L elements, in place).
 // `tw` is a flat dev-layout table holding at least 2^L - 1 entries.
 // `scratch` must hold 2^L dev-layout elements when L exceeds one tile.
+// `tw_one`: entry 0 of every stage of `tw` is 1 (tables of powers), which
+// lets the first stages skip multiplications by it.
 hipError_t launch_transform(int field, int L, uint32_t *data, uint32_t *scratch, const uint32_t *tw,
-                            const uint32_t *coset, const uint32_t *scale_dev, hipStream_t stream);
+                            const uint32_t *coset, const uint32_t *scale_dev, bool tw_one, hipStream_t stream);
 
 // Per-field pass launchers (pass_fr.hip, pass_gl.hip).
 hipError_t launch_transform_fr(int L, uint32_t *data, uint32_t *scratch, const uint32_t *tw, const uint32_t *coset,
-                               const uint32_t *scale_dev, hipStream_t stream);
+                               const uint32_t *scale_dev, bool tw_one, hipStream_t stream);
 hipError_t launch_transform_gl(int L, uint32_t *data, uint32_t *scratch, const uint32_t *tw, const uint32_t *coset,
-                               const uint32_t *scale_dev, hipStream_t stream);
+                               const uint32_t *scale_dev, bool tw_one, hipStream_t stream);
 
 // Twiddle table builders (dev layout).  `flat` holds 2^L entries: stages
 // 1..L at offsets 2^(s-1)-1 and a trailing one (index 2^L - 1).

@@ -61,10 +61,10 @@ PassPlan plan_passes_shape(int L, int tl, int kmax) {
 }
 
 hipError_t launch_transform(int field, int L, uint32_t *data, uint32_t *scratch, const uint32_t *tw,
-                            const uint32_t *coset, const uint32_t *scale_dev, hipStream_t stream) {
+                            const uint32_t *coset, const uint32_t *scale_dev, bool tw_one, hipStream_t stream) {
   if (L == 0) return hipSuccess;
-  if (field == kFr) return launch_transform_fr(L, data, scratch, tw, coset, scale_dev, stream);
-  return launch_transform_gl(L, data, scratch, tw, coset, scale_dev, stream);
+  if (field == kFr) return launch_transform_fr(L, data, scratch, tw, coset, scale_dev, tw_one, stream);
+  return launch_transform_gl(L, data, scratch, tw, coset, scale_dev, tw_one, stream);
 }
 
 // ---------------------------------------------------------------------------

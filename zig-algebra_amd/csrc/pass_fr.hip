@@ -4,14 +4,19 @@
 namespace zntt {
 
 hipError_t launch_transform_fr(int L, uint32_t *data, uint32_t *scratch, const uint32_t *tw, const uint32_t *coset,
-                               const uint32_t *scale_dev, hipStream_t stream) {
+                               const uint32_t *scale_dev, bool tw_one, hipStream_t stream) {
   static bool configured = false;
   if (!configured) {
     hipError_t e = configure_pass_lds<FrT>(9 * 4 * (1 << 11));
     if (e != hipSuccess) return e;
     configured = true;
   }
-  return run_passes_t<FrT>(kFr, L, data, scratch, tw, coset, scale_dev, stream);
+  return run_passes_t<FrT>(kFr, L, data, scratch, tw, coset, scale_dev, tw_one, stream);
+}
+
+hipError_t launch_whole_fr(int K, const BatchXform *xf, const BatchEntry *ent, uint32_t n, uint32_t *counter,
+                           hipStream_t s) {
+  return launch_whole_t<FrT>(K, xf, ent, n, counter, s);
 }
 
 }  // namespace zntt

@@ -7,8 +7,10 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
 #include <utility>
 
+#include "batch.hpp"
 #include "field_traits.hpp"
 #include "kernels.hpp"
 
@@ -238,8 +240,33 @@ __device__ __forceinline__ void pass_round(typename F::E (&x)[8], const PassArgs
     using M = StageMap<R, T>;
     sfor<2>([&](auto h) {
       constexpr int j0 = 2 * decltype(h)::value, j1 = j0 + 1;
-      F::template bfly2<norm>(x[M::u(j0)], x[M::u(j0) + (1 << T)], tw[M::set(j0) * (SET - 1) + (1 << T) - 1 + M::j(j0)],
-                              x[M::u(j1)], x[M::u(j1) + (1 << T)], tw[M::set(j1) * (SET - 1) + (1 << T) - 1 + M::j(j1)]);
+      auto both = [&] {
+        F::template bfly2<norm>(x[M::u(j0)], x[M::u(j0) + (1 << T)],
+                                tw[M::set(j0) * (SET - 1) + (1 << T) - 1 + M::j(j0)], x[M::u(j1)],
+                                x[M::u(j1) + (1 << T)], tw[M::set(j1) * (SET - 1) + (1 << T) - 1 + M::j(j1)]);
+      };
+      // In the first round of a transform's first pass the twiddle index of
+      // stage T is M::j, and index 0 is the twiddle 1: no product needed.
+      constexpr bool one0 = first_round && M::j(j0) == 0 && F::one_ok(T);
+      constexpr bool one1 = first_round && M::j(j1) == 0 && F::one_ok(T);
+      if constexpr (one0 || one1) {
+        if (a.first && a.tw_one) {
+          if constexpr (one0)
+            F::template bfly_one<norm>(x[M::u(j0)], x[M::u(j0) + (1 << T)]);
+          else
+            F::template bfly<norm>(x[M::u(j0)], x[M::u(j0) + (1 << T)],
+                                   tw[M::set(j0) * (SET - 1) + (1 << T) - 1 + M::j(j0)]);
+          if constexpr (one1)
+            F::template bfly_one<norm>(x[M::u(j1)], x[M::u(j1) + (1 << T)]);
+          else
+            F::template bfly<norm>(x[M::u(j1)], x[M::u(j1) + (1 << T)],
+                                   tw[M::set(j1) * (SET - 1) + (1 << T) - 1 + M::j(j1)]);
+        } else {
+          both();
+        }
+      } else {
+        both();
+      }
     });
   });
   // Store the sets (to the positions they came from, or to HBM).
@@ -312,6 +339,96 @@ __global__ __launch_bounds__(512) void ntt_pass_r8_kernel(const PassArgs a) {
   pass_rounds<F, K, 0, ODD>(x, a, lds, c);
 }
 
+// Persistent kernel over a batch of transforms that each fit one tile: item
+// i is transform ent[i].xf, run as a single pass of K = L stages with the
+// register rounds above (coalesced staging, compile-time indexing).  Items
+// are claimed from `*next`, which must be zero at launch; each workgroup
+// claims its following item before working on the current one.
+template <class F, int K>
+__global__ __launch_bounds__(512) void whole_batch_kernel(const BatchXform *xf, const BatchEntry *ent, uint32_t n,
+                                                          uint32_t *next) {
+  using E = typename F::E;
+  extern __shared__ uint32_t lds[];
+  __shared__ uint32_t s_item;
+  if (threadIdx.x == 0) s_item = atomicAdd(next, 1u);
+  __syncthreads();
+  uint32_t it = s_item;
+  while (it < n) {
+    uint32_t claimed = 0;
+    if (threadIdx.x == 0) claimed = atomicAdd(next, 1u);
+    const BatchXform &X = xf[ent[it].xf];
+    PassArgs a;
+    a.first = 1;
+    a.last = 1;
+    a.src = X.src;
+    a.dst = X.data;
+    a.src_len = X.src_len;
+    a.tw = X.tw;
+    a.coset = X.coset;
+    a.has_scale = X.has_scale;
+#pragma unroll
+    for (int k = 0; k < 9; k++) a.scale[k] = X.scale[k];
+    a.L = K;
+    a.s0 = 1;
+    a.K = K;
+    a.log_g = 0;
+    a.tw_one = 1;  // batches use tables of powers
+    TileCtx c;
+    c.K = K;
+    c.lg = 0;
+    c.G = 1;
+    c.tile = 1 << K;
+    c.lo_bits = 0;
+    c.hi_bits = 0;
+    c.hi = c.lo0 = c.c0 = 0;
+    E x[8];
+    pass_rounds<F, K, 0, true>(x, a, lds, c);
+    __syncthreads();  // LDS is reused by the next item
+    if (threadIdx.x == 0) s_item = claimed;
+    __syncthreads();
+    it = s_item;
+  }
+}
+
+template <class F, int K>
+static hipError_t launch_whole_k(const BatchXform *xf, const BatchEntry *ent, uint32_t n, uint32_t *counter,
+                                 hipStream_t s) {
+  static int grid = 0;  // resident workgroups on the whole device
+  const size_t lds = (size_t)F::W * 4 << K;
+  const unsigned threads = 1u << (K - 3);
+  if (grid == 0) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&whole_batch_kernel<F, K>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    int per_cu = 0, dev = 0, cus = 0;
+    if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
+    if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, whole_batch_kernel<F, K>, (int)threads, lds);
+    if (e != hipSuccess) return e;
+    grid = std::max(1, per_cu) * std::max(1, cus);
+  }
+  const unsigned g = (unsigned)std::min<uint32_t>(n, (uint32_t)grid);
+  hipLaunchKernelGGL((whole_batch_kernel<F, K>), dim3(g), dim3(threads), lds, s, xf, ent, n, counter);
+  return hipGetLastError();
+}
+
+// Transforms of 2^9 up to the field's largest register-round tile (BN254 Fr
+// 2^11, Goldilocks 2^12).
+template <class F>
+static hipError_t launch_whole_t(int K, const BatchXform *xf, const BatchEntry *ent, uint32_t n, uint32_t *counter,
+                                 hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  switch (K) {
+    case 9: return launch_whole_k<F, 9>(xf, ent, n, counter, s);
+    case 10: return launch_whole_k<F, 10>(xf, ent, n, counter, s);
+    case 11: return launch_whole_k<F, 11>(xf, ent, n, counter, s);
+    case 12:
+      if constexpr (F::W == 2) return launch_whole_k<F, 12>(xf, ent, n, counter, s);
+      return hipErrorInvalidValue;
+    default: return hipErrorInvalidValue;
+  }
+}
+
 // Pass lengths the planner produces for tiles of >= 512 elements: 5..12.
 template <class F>
 static hipError_t launch_r8(int K, bool odd, dim3 grid, dim3 block, size_t lds, hipStream_t s, const PassArgs &a) {
@@ -343,7 +460,7 @@ static hipError_t set_r8_lds(size_t bytes) {
 
 template <class F>
 static hipError_t run_passes_t(int field, int L, uint32_t *data, uint32_t *scratch, const uint32_t *tw,
-                             const uint32_t *coset, const uint32_t *scale_dev, hipStream_t stream) {
+                             const uint32_t *coset, const uint32_t *scale_dev, bool tw_one, hipStream_t stream) {
   const PassPlan plan = plan_passes(field, L);
   int s0 = 1;
   for (int i = 0; i < plan.passes; i++) {
@@ -359,6 +476,7 @@ static hipError_t run_passes_t(int field, int L, uint32_t *data, uint32_t *scrat
       for (int k = 0; k < F::W; k++) a.scale[k] = scale_dev[k];
     a.L = L;
     a.src_len = (uint64_t)1 << L;
+    a.tw_one = tw_one ? 1 : 0;
     a.s0 = s0;
     a.K = plan.K[i];
     a.log_g = plan.log_g[i];

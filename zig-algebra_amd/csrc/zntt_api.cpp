@@ -285,7 +285,7 @@ int check_transform_args(zntt_ctx *c, int field, const void *data, size_t len, s
 // Runs one transform over device data on stream s.  `tw` covers >= 2^L - 1
 // entries; `coset` (nullable) covers 2^L; `scale` (nullable) is a dev constant.
 int run_device(zntt_ctx *c, int field, int L, void *d_data, const uint32_t *tw, const uint32_t *coset,
-               const std::array<uint32_t, 9> *scale, hipStream_t s) {
+               const std::array<uint32_t, 9> *scale, hipStream_t s, bool tw_one = true) {
   if (L == 0) {
     if (!scale && !coset) return ZNTT_OK;
   }
@@ -295,7 +295,7 @@ int run_device(zntt_ctx *c, int field, int L, void *d_data, const uint32_t *tw, 
     if (st) return st;
   }
   hipError_t e = zntt::launch_transform(field, L, (uint32_t *)d_data, (uint32_t *)c->scratch, tw, coset,
-                                        scale ? scale->data() : nullptr, s);
+                                        scale ? scale->data() : nullptr, tw_one, s);
   return hip_status(e);
 }
 
@@ -365,6 +365,10 @@ int twiddled(zntt_ctx *c, int field, void *data, size_t len, size_t log_n, const
   if (log_n == 0) return ZNTT_OK;
   const int L = (int)log_n;
   const uint32_t *tw = nullptr;
+  // Registered tables are powers of a root, and derived inverse tables start
+  // every stage with 1; a foreign forward table is used as given, so stages
+  // may skip the multiplication by entry 0 only if each one holds 1.
+  bool tw_one = true;
   TwiddleEntry *reg = find_registered(c, field, log_n, stages);
   if (reg) {
     tw = reg->table;
@@ -383,6 +387,9 @@ int twiddled(zntt_ctx *c, int field, void *data, size_t len, size_t log_n, const
     st = upload_foreign(c, field, log_n, stages, &dev);
     if (st) return st;
     tw = dev;
+    const Elem one = f_one(field);
+    for (size_t s = 0; s < log_n && !inverse; s++)
+      if (!(load_elem(field, stages[s]) == one)) tw_one = false;
     if (inverse) {
       const size_t n = (size_t)1 << log_n;
       uint32_t *inv = (uint32_t *)((uint8_t *)c->aux + n * (dev_bytes(field) + user_bytes(field)));
@@ -394,7 +401,7 @@ int twiddled(zntt_ctx *c, int field, void *data, size_t len, size_t log_n, const
   std::array<uint32_t, 9> scale;
   if (inverse) scale = f_to_dev(field, f_inv(field, f_from_u64(field, len)));  // root.zig:208-211
   return with_staging(c, field, data, len, [&](void *d) {
-    return run_device(c, field, L, d, tw, nullptr, inverse ? &scale : nullptr, c->stream);
+    return run_device(c, field, L, d, tw, nullptr, inverse ? &scale : nullptr, c->stream, tw_one);
   });
 }
 

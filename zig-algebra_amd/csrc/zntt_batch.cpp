@@ -45,10 +45,14 @@ struct zntt_batch {
   size_t scratch_bytes = 0;
   void *dev = nullptr;                     // xf + entries + level counters
   size_t dev_bytes = 0;
-  uint32_t level_ent[zntt::kBatchMaxPasses + 1] = {};  // entries of level lv: [level_ent[lv], level_ent[lv+1])
-  uint32_t level_items[zntt::kBatchMaxPasses] = {};
-  uint32_t level_uniform[zntt::kBatchMaxPasses] = {};  // tiles per entry when all equal, else 0
-  int levels = 0;
+  // The launches of one execution, in order.  whole_log 0: the generic
+  // kernel over a pass level; else the whole-transform kernel for
+  // single-pass transforms of 2^whole_log elements.
+  struct Launch {
+    int whole_log;
+    uint32_t ent_off, nent, nitems, uniform;
+  };
+  std::vector<Launch> launches;
   zntt::BatchXform *d_xf = nullptr;
   zntt::BatchEntry *d_ent = nullptr;
   uint32_t *d_counters = nullptr;
@@ -101,26 +105,38 @@ int build_plan(zntt_batch *b, const Shape &shape) {
   if (shape.order == 1)
     std::stable_sort(idx.begin(), idx.end(), [&](uint32_t u, uint32_t v) { return b->xf[u].L > b->xf[v].L; });
   std::vector<zntt::BatchEntry> ent;
-  b->levels = levels;
-  b->level_ent[0] = 0;
+  b->launches.clear();
+  // Transforms that fit one register-round tile go to the whole-transform
+  // kernel, one launch per size; the rest run level by level.
+  auto whole = [&](const zntt::BatchXform &x) {
+    return x.npass == 1 && x.L >= zntt::kWholeMinLog && x.L <= zntt::whole_max_log(b->field);
+  };
   for (int p = 0; p < levels; p++) {
     uint32_t first = 0, uniform = 0;
     bool equal = true;
+    const uint32_t off = (uint32_t)ent.size();
     for (uint32_t t : idx) {
       const zntt::BatchXform &x = b->xf[t];
-      if (p >= x.npass) continue;
+      if (p >= x.npass || whole(x)) continue;
       ent.push_back(zntt::BatchEntry{t, (uint32_t)p, first, 0});
       first += x.tiles[p];
       if (uniform == 0) uniform = x.tiles[p];
       equal = equal && uniform == x.tiles[p];
     }
-    b->level_ent[p + 1] = (uint32_t)ent.size();
-    b->level_items[p] = first;
-    b->level_uniform[p] = equal ? uniform : 0;
+    if (first) b->launches.push_back({0, off, (uint32_t)ent.size() - off, first, equal ? uniform : 0});
+    if (p == 0) {
+      for (int L = zntt::whole_max_log(b->field); L >= zntt::kWholeMinLog; L--) {
+        const uint32_t woff = (uint32_t)ent.size();
+        uint32_t k = 0;
+        for (uint32_t t : idx)
+          if (whole(b->xf[t]) && b->xf[t].L == L) ent.push_back(zntt::BatchEntry{t, 0, k++, 0});
+        if (k) b->launches.push_back({L, woff, k, k, 1});
+      }
+    }
   }
   const size_t xf_bytes = b->xf.size() * sizeof(zntt::BatchXform);
   const size_t it_bytes = ent.size() * sizeof(zntt::BatchEntry);
-  const size_t st_bytes = zntt::kBatchMaxPasses * sizeof(uint32_t);
+  const size_t st_bytes = b->launches.size() * sizeof(uint32_t);
   auto up = [](size_t v) { return (v + 255) & ~(size_t)255; };
   b->dev_bytes = up(xf_bytes) + up(it_bytes) + up(st_bytes);
   int st = dev_alloc(c, b->dev_bytes, &b->dev);
@@ -141,13 +157,19 @@ int build_plan(zntt_batch *b, const Shape &shape) {
   return ZNTT_OK;
 }
 
-// Clears the level counters, then one persistent launch per pass level.
+// Clears the launch counters, then the persistent launches in order.
 hipError_t enqueue(zntt_batch *b, hipStream_t s) {
-  hipError_t e = zntt::launch_clear_counters(b->d_counters, (uint32_t)b->levels, s);
-  for (int lv = 0; lv < b->levels && e == hipSuccess; lv++)
-    e = zntt::launch_batch_level(b->field, b->d_xf, b->d_ent + b->level_ent[lv],
-                                 b->level_ent[lv + 1] - b->level_ent[lv], b->level_items[lv],
-                                 b->level_uniform[lv], b->d_counters + lv, b->grid, b->shape.tile_log, s);
+  hipError_t e = zntt::launch_clear_counters(b->d_counters, (uint32_t)b->launches.size(), s);
+  for (size_t i = 0; i < b->launches.size() && e == hipSuccess; i++) {
+    const zntt_batch::Launch &l = b->launches[i];
+    if (l.whole_log == 0)
+      e = zntt::launch_batch_level(b->field, b->d_xf, b->d_ent + l.ent_off, l.nent, l.nitems, l.uniform,
+                                   b->d_counters + i, b->grid, b->shape.tile_log, s);
+    else if (b->field == ZNTT_FIELD_BN254_FR)
+      e = zntt::launch_whole_fr(l.whole_log, b->d_xf, b->d_ent + l.ent_off, l.nent, b->d_counters + i, s);
+    else
+      e = zntt::launch_whole_gl(l.whole_log, b->d_xf, b->d_ent + l.ent_off, l.nent, b->d_counters + i, s);
+  }
   return e;
 }
 
