@@ -64,7 +64,8 @@ enum zntt_status {
   ZNTT_NO_DEVICE = 7,         /* zntt_ctx_create: no such HIP device             */
   ZNTT_INVALID_ARGUMENT = 8,  /* null context/pointer, unknown field id          */
   ZNTT_UNSUPPORTED = 9,       /* a size this device path does not implement      */
-  ZNTT_ORDER_TOO_LARGE = 10   /* error.OrderTooLarge    roots.zig:150 (LDE/roots) */
+  ZNTT_ORDER_TOO_LARGE = 10,  /* error.OrderTooLarge    roots.zig:150 (LDE/roots) */
+  ZNTT_DEGREE_TOO_LARGE = 11  /* error.DegreeTooLarge   libs/poly/src/poly.zig:200 */
 };
 
 enum zntt_field {
@@ -325,6 +326,31 @@ void zntt_server_destroy(zntt_server *s);
  * element i of (seed, stream_id) for i < n, in the field's storage layout. */
 int zntt_fill_counter(zntt_ctx *ctx, int field, uint64_t seed, uint64_t stream_id, void *data, size_t n,
                       void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* Polynomial multiplication by NTT.  Replaces the O(n^2) schoolbook product */
+/* of libs/poly (Polynomial.mul, poly.zig:197-212) with forward transforms   */
+/* of both operands zero-padded to 2^L >= deg a + deg b + 1, a pointwise     */
+/* product and an inverse transform; the cyclic product of that length is   */
+/* the linear one, so the coefficients are exactly the schoolbook ones.      */
+/* ------------------------------------------------------------------------ */
+
+/* Polynomial.mul on coefficient arrays (host buffers, coefficient i of x^i,
+ * storage layout).  Degrees are those of the arrays with trailing zero
+ * coefficients removed, as the reference's normalised Polynomial holds them.
+ * A zero operand gives the zero polynomial (*out_len = 0, poly.zig:198).
+ * ZNTT_DEGREE_TOO_LARGE when deg a + deg b + 1 > out_cap (the reference's
+ * max_degree + 1, poly.zig:200); ZNTT_ORDER_TOO_LARGE when the padded size
+ * exceeds the field's two-adicity; ZNTT_UNSUPPORTED beyond ZNTT_MAX_LOG_N.
+ * On success out[0 .. *out_len) holds the product. */
+int zntt_poly_mul(zntt_ctx *ctx, int field, const void *a, size_t a_len, const void *b, size_t b_len, void *out,
+                  size_t out_cap, size_t *out_len);
+
+/* The same on device buffers, enqueued on `stream`: out[0 .. a_len + b_len - 1)
+ * = a * b with the operands taken at their full lengths (trailing zeros give
+ * zero coefficients).  Uses context scratch of 2 * 2^L elements. */
+int zntt_poly_mul_device(zntt_ctx *ctx, int field, const void *a, size_t a_len, const void *b, size_t b_len,
+                         void *out, void *stream);
 
 /* Blocks until every call enqueued on the context's own stream is done. */
 int zntt_ctx_synchronize(zntt_ctx *ctx);

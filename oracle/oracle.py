@@ -38,6 +38,7 @@ INVALID_TWIDDLES = 4
 OUT_OF_MEMORY = 5
 INVALID_ARGUMENT = 8
 ORDER_TOO_LARGE = 10
+DEGREE_TOO_LARGE = 11
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "liboracle.so")
@@ -73,6 +74,8 @@ def lib():
         L.oracle_lde.argtypes = [i32, vp, sz, sz, vp, vp]
         L.oracle_gen_uniform.argtypes = [i32, u64, vp, sz]
         L.oracle_fill_counter.argtypes = [i32, u64, u64, vp, sz]
+        L.oracle_poly_mul.argtypes = [i32, vp, sz, vp, sz, vp, sz, ctypes.POINTER(sz)]
+        L.oracle_poly_eval.argtypes = [i32, vp, sz, vp, vp]
         L.oracle_xoshiro_stream.argtypes = [u64, vp, sz]
         L.oracle_xoshiro_stream.restype = None
         L.oracle_ntt_batch_mt.argtypes = [i32, vp, vp, vp, vp, sz, i32]
@@ -212,6 +215,24 @@ def lde(field: int, coeffs: np.ndarray, log_k: int, log_blowup: int, shift: np.n
 def gen_uniform(field: int, seed: int, n: int) -> np.ndarray:
     out = empty(field, n)
     st = lib().oracle_gen_uniform(field, seed, _ptr(out), n)
+    assert st == OK
+    return out
+
+
+def poly_mul(field: int, a: np.ndarray, b: np.ndarray, cap: int | None = None) -> tuple[int, np.ndarray]:
+    """Schoolbook Polynomial.mul (libs/poly/src/poly.zig:197-212): (status, product)."""
+    cap = a.shape[0] + b.shape[0] if cap is None else cap
+    out = empty(field, max(cap, 1))
+    n = ctypes.c_size_t()
+    st = lib().oracle_poly_mul(field, _ptr(np.ascontiguousarray(a)), a.shape[0], _ptr(np.ascontiguousarray(b)),
+                               b.shape[0], _ptr(out), cap, ctypes.byref(n))
+    return st, out[:n.value].copy()
+
+
+def poly_eval(field: int, coeffs: np.ndarray, point) -> np.ndarray:
+    out = empty(field, 1).reshape(-1)
+    st = lib().oracle_poly_eval(field, _ptr(np.ascontiguousarray(coeffs)), coeffs.shape[0],
+                                _ptr(_as_elem(field, point)), _ptr(out))
     assert st == OK
     return out
 

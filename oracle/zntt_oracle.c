@@ -51,6 +51,7 @@ enum {
   OR_OUT_OF_MEMORY = 5,
   OR_ORDER_TOO_LARGE = 10,
   OR_INVALID_ARGUMENT = 8,
+  OR_DEGREE_TOO_LARGE = 11,
 };
 
 /* ------------------------------------------------------------------------ */
@@ -815,6 +816,78 @@ int oracle_gen_uniform(int field, uint64_t seed, void *out, size_t n) {
   if (!sf_for(field, &c)) return OR_INVALID_ARGUMENT;
   uint64_t *o = (uint64_t *)out;
   for (size_t i = 0; i < n; i++) o[i] = sf_draw(&c, &x);
+  return OR_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Polynomials over the field (libs/poly/src/poly.zig).  Coefficient i is   */
+/* the coefficient of x^i; the degree is that of the array without trailing */
+/* zero coefficients, as the reference's normalised Polynomial keeps it.     */
+/* ------------------------------------------------------------------------ */
+
+static size_t poly_len(int field, const unsigned char *p, size_t n, int es) {
+  while (n > 0) {
+    const unsigned char *e = p + (n - 1) * (size_t)es;
+    int zero = 1;
+    for (int i = 0; i < es; i++) zero &= e[i] == 0;
+    if (!zero) break;
+    n--;
+  }
+  (void)field;
+  return n;
+}
+
+/* Polynomial.mul, poly.zig:197-212: zero if either operand is zero (:198);
+ * error.DegreeTooLarge if deg a + deg b > max_degree, here out_cap - 1
+ * (:199-200); else r[i + j] += a[i] * b[j] over both degrees (:203-208). */
+int oracle_poly_mul(int field, const void *a, size_t a_len, const void *b, size_t b_len, void *out,
+                    size_t out_cap, size_t *out_len) {
+  fr_init();
+  int es = oracle_elem_size(field);
+  if (!es) return OR_INVALID_ARGUMENT;
+  *out_len = 0;
+  size_t na = poly_len(field, (const unsigned char *)a, a_len, es);
+  size_t nb = poly_len(field, (const unsigned char *)b, b_len, es);
+  if (na == 0 || nb == 0) return OR_OK;
+  size_t m = na + nb - 1;
+  if (m > out_cap) return OR_DEGREE_TOO_LARGE;
+  if (field == FIELD_FR) {
+    const fr_t *x = (const fr_t *)a, *y = (const fr_t *)b;
+    fr_t *r = (fr_t *)out;
+    for (size_t k = 0; k < m; k++) r[k] = fr_zero();
+    for (size_t i = 0; i < na; i++)
+      for (size_t j = 0; j < nb; j++) r[i + j] = fr_add(r[i + j], fr_mul(x[i], y[j]));
+  } else {
+    sf_ctx c;
+    if (!sf_for(field, &c)) return OR_INVALID_ARGUMENT;
+    const sf_t *x = (const sf_t *)a, *y = (const sf_t *)b;
+    sf_t *r = (sf_t *)out;
+    for (size_t k = 0; k < m; k++) r[k].v = 0;
+    for (size_t i = 0; i < na; i++)
+      for (size_t j = 0; j < nb; j++) r[i + j] = sf_add(&c, r[i + j], sf_mul(&c, x[i], y[j]));
+  }
+  *out_len = m;
+  return OR_OK;
+}
+
+/* Polynomial.eval, poly.zig:215-224 (Horner from the leading coefficient). */
+int oracle_poly_eval(int field, const void *coeffs, size_t n, const void *point, void *out) {
+  fr_init();
+  if (field == FIELD_FR) {
+    const fr_t *c = (const fr_t *)coeffs;
+    fr_t z, r = fr_zero();
+    memcpy(&z, point, 32);
+    for (size_t i = n; i-- > 0;) r = fr_add(fr_mul(r, z), c[i]);
+    memcpy(out, &r, 32);
+    return OR_OK;
+  }
+  sf_ctx cx;
+  if (!sf_for(field, &cx)) return OR_INVALID_ARGUMENT;
+  const sf_t *c = (const sf_t *)coeffs;
+  sf_t z, r = {0};
+  memcpy(&z.v, point, 8);
+  for (size_t i = n; i-- > 0;) r = sf_add(&cx, sf_mul(&cx, r, z), c[i]);
+  memcpy(out, &r.v, 8);
   return OR_OK;
 }
 

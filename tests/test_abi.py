@@ -45,8 +45,9 @@ def test_status_codes_match_oracle_and_reference_error_names(Z):
     assert Z.InvalidTwiddles.status == O.INVALID_TWIDDLES
     assert Z.OutOfMemory.status == O.OUT_OF_MEMORY
     assert Z.OrderTooLarge.status == O.ORDER_TOO_LARGE
+    assert Z.DegreeTooLarge.status == O.DEGREE_TOO_LARGE
     for cls in (Z.LogTooLarge, Z.LengthMismatch, Z.InvalidLength, Z.InvalidTwiddles, Z.OutOfMemory,
-                Z.OrderTooLarge):
+                Z.OrderTooLarge, Z.DegreeTooLarge):
         assert Z.lib.zntt_status_string(cls.status).decode() == cls.__name__
 
 

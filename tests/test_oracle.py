@@ -323,3 +323,65 @@ def test_textbook_digests_2p20():
     st, tw = O.precompute_twiddles(F, 20, root)
     assert O.ntt_with_twiddles(F, data, 20, tw) == O.OK
     assert hashlib.sha256(data.tobytes()).hexdigest() == c["forward_digest"]
+
+
+# ---------------------------------------------------------------------------
+# Polynomial product (libs/poly/src/poly.zig:197-224), pinned to Python ints
+# ---------------------------------------------------------------------------
+
+def schoolbook(p, a, b):
+    """Python-int product of coefficient lists, after removing trailing zeros."""
+    while a and a[-1] == 0:
+        a = a[:-1]
+    while b and b[-1] == 0:
+        b = b[:-1]
+    if not a or not b:
+        return []
+    r = [0] * (len(a) + len(b) - 1)
+    for i, x in enumerate(a):
+        for j, y in enumerate(b):
+            r[i + j] = (r[i + j] + x * y) % p
+    return r
+
+
+@pytest.mark.parametrize("name", ["fr", "goldilocks", "babybear"])
+def test_poly_mul_matches_python_schoolbook(name):
+    F = FIELDS[name]
+    p = O.MODULUS[F]
+    rng = np.random.default_rng(5)
+    cases = [(0, 0), (1, 0), (1, 1), (3, 5), (17, 1), (33, 31)]
+    for la, lb in cases:
+        a = [int(v) for v in rng.integers(0, 1 << 62, la)]
+        b = [int(v) for v in rng.integers(0, 1 << 62, lb)]
+        if la > 2:
+            a[-1] = 0  # a trailing zero is not part of the degree
+        st, got = O.poly_mul(F, elems(F, a), elems(F, b))
+        assert st == O.OK
+        assert canon(F, got) == schoolbook(p, [x % p for x in a], [x % p for x in b]), (la, lb)
+
+
+def test_poly_mul_degree_too_large_and_zero():
+    F = O.FIELD_GL
+    a = elems(F, [1, 2, 3])  # degree 2
+    b = elems(F, [4, 5])     # degree 1
+    st, got = O.poly_mul(F, a, b, cap=3)
+    assert st == O.DEGREE_TOO_LARGE and got.shape[0] == 0
+    st, got = O.poly_mul(F, a, b, cap=4)
+    assert st == O.OK and canon(F, got) == [4, 13, 22, 15]
+    st, got = O.poly_mul(F, a, elems(F, [0, 0]), cap=0)
+    assert st == O.OK and got.shape[0] == 0  # zero polynomial wins over capacity
+
+
+def test_poly_eval_matches_python_and_product_identity():
+    for F in (O.FIELD_FR, O.FIELD_GL):
+        p = O.MODULUS[F]
+        a = O.gen_uniform(F, 41, 50)
+        b = O.gen_uniform(F, 42, 70)
+        z = 123456789123456789 % p
+        ai, bi = canon(F, a), canon(F, b)
+        assert O.value(F, O.poly_eval(F, a, O.from_int(F, z))) == sum(c * pow(z, i, p) for i, c in enumerate(ai)) % p
+        st, r = O.poly_mul(F, a, b)
+        assert st == O.OK
+        lhs = O.value(F, O.poly_eval(F, r, O.from_int(F, z)))
+        rhs = (sum(c * pow(z, i, p) for i, c in enumerate(ai)) * sum(c * pow(z, i, p) for i, c in enumerate(bi))) % p
+        assert lhs == rhs

@@ -71,6 +71,10 @@ hipError_t launch_table_to_user(int field, size_t count, uint32_t *user, const u
 hipError_t launch_inverse_table(int field, int L, uint32_t *inv, const uint32_t *fwd, hipStream_t stream);
 // In-place bit-reversal permutation of n = 2^L user-layout elements.
 hipError_t launch_bit_reverse(int field, int L, uint32_t *data, hipStream_t stream);
+// a[i] <- a[i] * b[i] for i < n, both in the caller's layout.  BN254 Fr:
+// the Montgomery product of two data-form values is x*y*2^251 (R' = 2^261),
+// which a later scale by c*2^5 (dev form) returns to the 2^256 form.
+hipError_t launch_pointwise_mul(int field, uint32_t *a, const uint32_t *b, size_t n, hipStream_t stream);
 // Elementwise device field op (0 add, 1 sub, 2 mul) on user-layout arrays.
 hipError_t launch_field_op(int field, int op, size_t n, const uint32_t *a, const uint32_t *b, uint32_t *out,
                            hipStream_t stream);

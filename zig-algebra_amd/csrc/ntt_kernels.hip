@@ -25,7 +25,10 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "field_traits.hpp"
+
 #include "kernels.hpp"
 
 namespace zntt {
@@ -231,6 +234,22 @@ __global__ void bit_reverse_kernel(uint32_t *data, int L) {
     a[k] = b[k];
     b[k] = t;
   }
+}
+
+template <class F>
+__global__ void pointwise_mul_kernel(uint32_t *a, const uint32_t *b, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    F::store_user(a, i, F::mul(F::load_user(a, i), F::load_user(b, i)));
+}
+
+hipError_t launch_pointwise_mul(int field, uint32_t *a, const uint32_t *b, size_t n, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 65536);
+  if (field == kFr)
+    hipLaunchKernelGGL(pointwise_mul_kernel<FrT>, dim3(blocks), dim3(256), 0, stream, a, b, n);
+  else
+    hipLaunchKernelGGL(pointwise_mul_kernel<GlT>, dim3(blocks), dim3(256), 0, stream, a, b, n);
+  return hipGetLastError();
 }
 
 hipError_t launch_bit_reverse(int field, int L, uint32_t *data, hipStream_t stream) {

@@ -482,6 +482,7 @@ const char *zntt_status_string(int status) {
     case ZNTT_INVALID_ARGUMENT: return "InvalidArgument";
     case ZNTT_UNSUPPORTED: return "Unsupported";
     case ZNTT_ORDER_TOO_LARGE: return "OrderTooLarge";
+    case ZNTT_DEGREE_TOO_LARGE: return "DegreeTooLarge";
     default: return "unknown";
   }
 }
@@ -641,6 +642,72 @@ int zntt_coset_ntt(zntt_ctx *c, int field, void *data, size_t len, size_t log_n,
   });
 }
 
+}  // extern "C"
+
+namespace {
+
+// Degree + 1 of a coefficient array: its length without trailing zeros.
+size_t effective_len(int field, const void *p, size_t n) {
+  const size_t ub = user_bytes(field);
+  const uint8_t *b = (const uint8_t *)p;
+  while (n > 0) {
+    const uint8_t *e = b + (n - 1) * ub;
+    bool zero = true;
+    for (size_t i = 0; i < ub && zero; i++) zero = e[i] == 0;  // zero is all-zero bytes in both layouts
+    if (!zero) break;
+    n--;
+  }
+  return n;
+}
+
+// Product of a (na coefficients) and b (nb) into out (na + nb - 1), all on
+// the device, on stream s: pad to 2^L, forward both, pointwise product,
+// inverse with the n^-1 scale (times 2^5 for BN254 Fr's data x data
+// Montgomery product).  Caller holds the context lock.
+int poly_mul_device(zntt_ctx *c, int field, const void *a, size_t na, const void *b, size_t nb, void *out,
+                    hipStream_t s) {
+  const size_t m = na + nb - 1;
+  int L = 1;  // two constants are multiplied as a 2-point cyclic product
+  while (((size_t)1 << L) < m) L++;
+  if (L > two_adicity(field)) return ZNTT_ORDER_TOO_LARGE;
+  if (L > ZNTT_MAX_LOG_N) return ZNTT_UNSUPPORTED;
+  const size_t n = (size_t)1 << L, ub = user_bytes(field);
+  int st = ensure(c, &c->staging, &c->staging_bytes, 2 * n * ub);
+  if (st) return st;
+  uint8_t *A = (uint8_t *)c->staging, *B = A + n * ub;
+  hipError_t e = hipMemcpyAsync(A, a, na * ub, hipMemcpyDeviceToDevice, s);
+  if (e == hipSuccess) e = hipMemsetAsync(A + na * ub, 0, (n - na) * ub, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(B, b, nb * ub, hipMemcpyDeviceToDevice, s);
+  if (e == hipSuccess) e = hipMemsetAsync(B + nb * ub, 0, (n - nb) * ub, s);
+  if (e != hipSuccess) return hip_status(e);
+  const Elem root = f_primitive_root(field, L);
+  std::array<uint32_t, 9> scale;
+  Elem sc = f_inv(field, f_from_u64(field, n));
+  if (field == ZNTT_FIELD_BN254_FR) sc = f_mul(field, sc, f_from_u64(field, 32));
+  scale = f_to_dev(field, sc);
+  {
+    TwiddleEntry *fw = nullptr, *inv = nullptr;
+    st = power_table(c, field, root, L, s, &fw);
+    if (st) return st;
+    fw->pins++;
+    st = power_table(c, field, f_inv(field, root), L, s, &inv);
+    fw->pins--;
+    if (st) return st;
+    st = run_device(c, field, L, A, fw->table, nullptr, nullptr, s);
+    if (st == ZNTT_OK) st = run_device(c, field, L, B, fw->table, nullptr, nullptr, s);
+    if (st) return st;
+    e = zntt::launch_pointwise_mul(field, (uint32_t *)A, (const uint32_t *)B, n, s);
+    if (e != hipSuccess) return hip_status(e);
+    st = run_device(c, field, L, A, inv->table, nullptr, &scale, s);
+    if (st) return st;
+  }
+  return hip_status(hipMemcpyAsync(out, A, m * ub, hipMemcpyDeviceToDevice, s));
+}
+
+}  // namespace
+
+extern "C" {
+
 int zntt_lde(zntt_ctx *c, int field, const void *coeffs, size_t log_k, size_t log_blowup, const void *shift,
              void *out) {
   if (!c || !field_ok(field) || !coeffs || !shift || !out) return ZNTT_INVALID_ARGUMENT;
@@ -675,6 +742,41 @@ int zntt_lde(zntt_ctx *c, int field, const void *coeffs, size_t log_k, size_t lo
   e = hipMemcpyAsync(out, c->staging, n * ub, hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   return hip_status(e);
+}
+
+int zntt_poly_mul(zntt_ctx *c, int field, const void *a, size_t a_len, const void *b, size_t b_len, void *out,
+                  size_t out_cap, size_t *out_len) {
+  if (!c || !field_ok(field) || !out_len || (!a && a_len) || (!b && b_len)) return ZNTT_INVALID_ARGUMENT;
+  *out_len = 0;
+  const size_t na = effective_len(field, a, a_len), nb = effective_len(field, b, b_len);
+  if (na == 0 || nb == 0) return ZNTT_OK;  // the zero polynomial, poly.zig:198
+  const size_t m = na + nb - 1;
+  if (m > out_cap) return ZNTT_DEGREE_TOO_LARGE;  // poly.zig:199-200
+  if (!out) return ZNTT_INVALID_ARGUMENT;
+  const size_t ub = user_bytes(field);
+  CallGuard g(c);
+  void *dev = nullptr;
+  int st = ensure(c, &c->aux, &c->aux_bytes, (na + nb + m) * ub);
+  if (st) return st;
+  dev = c->aux;
+  uint8_t *da = (uint8_t *)dev, *db = da + na * ub, *dout = db + nb * ub;
+  hipError_t e = hipMemcpyAsync(da, a, na * ub, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(db, b, nb * ub, hipMemcpyHostToDevice, c->stream);
+  if (e != hipSuccess) return hip_status(e);
+  st = poly_mul_device(c, field, da, na, db, nb, dout, c->stream);
+  if (st) return st;
+  e = hipMemcpyAsync(out, dout, m * ub, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return hip_status(e);
+  *out_len = m;
+  return ZNTT_OK;
+}
+
+int zntt_poly_mul_device(zntt_ctx *c, int field, const void *a, size_t a_len, const void *b, size_t b_len,
+                         void *out, void *stream) {
+  if (!c || !field_ok(field) || !a || !b || !out || a_len == 0 || b_len == 0) return ZNTT_INVALID_ARGUMENT;
+  CallGuard g(c);
+  return poly_mul_device(c, field, a, a_len, b, b_len, out, (hipStream_t)stream);
 }
 
 int zntt_execute_device(zntt_ctx *c, const zntt_transform *t, void *stream) {

@@ -75,8 +75,13 @@ class OrderTooLarge(ZnttError):
     status = 10
 
 
+class DegreeTooLarge(ZnttError):
+    status = 11
+
+
 _ERRORS = {c.status: c for c in (LogTooLarge, LengthMismatch, InvalidLength, InvalidTwiddles, OutOfMemory,
-                                 DeviceError, NoDevice, InvalidArgument, Unsupported, OrderTooLarge)}
+                                 DeviceError, NoDevice, InvalidArgument, Unsupported, OrderTooLarge,
+                                 DegreeTooLarge)}
 
 # Every symbol include/zntt_gpu.h declares.
 EXPORTED = [
@@ -90,6 +95,7 @@ EXPORTED = [
     "zntt_server_create", "zntt_server_submit", "zntt_server_query", "zntt_server_wait", "zntt_server_cancel",
     "zntt_server_read", "zntt_server_release", "zntt_server_drain", "zntt_server_stats_get", "zntt_server_sim_pump",
     "zntt_server_sim_complete", "zntt_server_destroy", "zntt_fill_counter",
+    "zntt_poly_mul", "zntt_poly_mul_device",
 ]
 
 REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED, REQ_CANCELLED = 0, 1, 2, 3, 4
@@ -194,6 +200,8 @@ def _load():
     L.zntt_server_destroy.argtypes = [vp]
     L.zntt_server_destroy.restype = None
     L.zntt_fill_counter.argtypes = [vp, i32, u64, u64, vp, sz, vp]
+    L.zntt_poly_mul.argtypes = [vp, i32, vp, sz, vp, sz, vp, sz, ctypes.POINTER(sz)]
+    L.zntt_poly_mul_device.argtypes = [vp, i32, vp, sz, vp, sz, vp, vp]
     return L
 
 
@@ -323,6 +331,22 @@ class Context:
     def coset_ntt(self, field: int, data: np.ndarray, log_n: int, root, shift) -> None:
         check(lib.zntt_coset_ntt(self._h, field, _ptr(data), data.shape[0], log_n,
                                  _ptr(_elem(field, root)), _ptr(_elem(field, shift))))
+
+    def poly_mul(self, field: int, a: np.ndarray, b: np.ndarray, cap: int | None = None) -> np.ndarray:
+        """Polynomial.mul (libs/poly/src/poly.zig:197-212) by NTT; `cap` is
+        the reference's max_degree + 1 (default: big enough)."""
+        cap = a.shape[0] + b.shape[0] if cap is None else cap
+        out = empty(field, max(cap, 1))
+        n = ctypes.c_size_t()
+        check(lib.zntt_poly_mul(self._h, field, _ptr(np.ascontiguousarray(a)), a.shape[0],
+                                _ptr(np.ascontiguousarray(b)), b.shape[0], _ptr(out), cap, ctypes.byref(n)))
+        return out[:n.value].copy()
+
+    def poly_mul_device(self, field: int, a_ptr: int, a_len: int, b_ptr: int, b_len: int, out_ptr: int,
+                        stream: int | None = None) -> None:
+        if stream is None:
+            stream = lib.zntt_ctx_stream(self._h)
+        check(lib.zntt_poly_mul_device(self._h, field, a_ptr, a_len, b_ptr, b_len, out_ptr, stream))
 
     def lde(self, field: int, coeffs: np.ndarray, log_k: int, log_blowup: int, shift) -> np.ndarray:
         out = empty(field, 1 << (log_k + log_blowup))
