@@ -13,6 +13,8 @@ Other configurations print their own JSON line when asked for:
                         coset-forward, one persistent launch vs one call each
   --workload ref-2p20   the reference's own NTT benchmark: nttWithTwiddles over
                         2^20 BN254 Fr, input fromInt(i % 1000), in place
+  --workload poly-mul   product of two 2^20-coefficient BN254 Fr polynomials by
+                        NTT (the widening of libs/poly's schoolbook mul)
   --workload serve      the coset-LDE serving loop: Poisson arrivals replayed
                         against zntt_server_* (continuous batching, HBM budget),
                         p50/p99 latency beside throughput, plus a run at a
@@ -320,6 +322,61 @@ def run_ref_2p20(args, Z, ctx):
     return line
 
 
+def run_poly_mul(args, Z, ctx):
+    """Polynomial product by NTT (the widening of libs/poly's schoolbook
+    Polynomial.mul, poly.zig:197-212): two BN254 Fr polynomials of 2^20
+    coefficients each, resident in HBM, product of 2^21 - 1 coefficients."""
+    import numpy as np
+    import torch
+    F, k = Z.FIELD_FR, 20
+    n = 1 << k
+    a = Z.fill_uniform(F, 11, n)
+    b = Z.fill_uniform(F, 12, n)
+    stream = torch.cuda.current_stream()
+    sp = stream.cuda_stream
+    ta = torch.from_numpy(a.view(np.int64)).cuda()
+    tb = torch.from_numpy(b.view(np.int64)).cuda()
+    out = torch.zeros((2 * n - 1) * 4, dtype=torch.int64, device="cuda")
+    ms = timed(lambda: ctx.poly_mul_device(F, ta.data_ptr(), n, tb.data_ptr(), n, out.data_ptr(), stream=sp),
+               args.steps, args.warmup, stream)
+    torch.cuda.synchronize()
+    r = out.cpu().numpy().view(np.uint64).reshape(2 * n - 1, 4)
+    import oracle as O
+    p = O.MODULUS[O.FIELD_FR]
+    ok = True
+    for z in (7, 987654321):
+        zz = O.from_int(O.FIELD_FR, z)
+        lhs = O.value(O.FIELD_FR, O.poly_eval(O.FIELD_FR, r, zz))
+        rhs = O.value(O.FIELD_FR, O.poly_eval(O.FIELD_FR, a, zz)) * O.value(O.FIELD_FR, O.poly_eval(O.FIELD_FR, b, zz))
+        ok = ok and lhs == rhs % p
+    L = k + 1
+    N = 1 << L
+    muls = (3 * (N // 2) * L + 2 * N) * MULS_PER_MODMUL  # three transforms, pointwise product, n^-1 scale
+    achieved = muls / (ms * 1e-3)
+    roofline = {"bound": "valu", "achieved": achieved / 1e12, "peak": VALU_MUL_PEAK / 1e12, "unit": "Tmul32/s",
+                "frac": achieved / VALU_MUL_PEAK, "traffic": None}
+    cpu = None
+    if not args.skip_cpu:
+        m = 1 << 13
+        t1 = time.time()
+        st, _ = O.poly_mul(O.FIELD_FR, O.gen_uniform(O.FIELD_FR, 11, m), O.gen_uniform(O.FIELD_FR, 12, m))
+        cs = time.time() - t1
+        assert st == O.OK
+        cpu = {"value": (2 * m - 1) / cs / 1e6, "unit": "Mcoeff/s", "cores": 1, "kind": "port",
+               "sample": f"schoolbook product of two 2^13-coefficient polynomials (the reference's O(n^2) "
+                         f"algorithm), C oracle, 1 thread, {cs:.1f} s; at 2^20 it would take "
+                         f"{cs * (n / m) ** 2 / 3600:.0f} h"}
+    line = base_line("Mcoeff/s BN254 Fr polynomial product 2^20 x 2^20 (NTT)", (2 * n - 1) / (ms * 1e-3) / 1e6,
+                     "Mcoeff/s", args, ms,
+                     {"workload": "product of two 2^20-coefficient BN254 Fr polynomials, resident in HBM "
+                                  "(forward NTTs of 2^21, pointwise product, inverse NTT)", "field": "bn254_fr",
+                      "coefficients": n},
+                     "synthetic: uniform BN254 Fr coefficients, seeds 11 and 12", "u32")
+    line.update({"roofline": roofline, "cpu_baseline": cpu,
+                 "extras": {"evaluation_identity_holds": ok, "transform_log_n": L}})
+    return line
+
+
 def serve_muls(arrivals):
     """Algorithmic 32-bit multiplies of a schedule: per request (N/2) log2 N
     butterfly products plus 2^k coset products, N = 2^(k+b)."""
@@ -406,7 +463,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="fr-ntt",
-                    choices=["fr-ntt", "gl-batch", "fr-uneven", "serve", "ref-2p20"])
+                    choices=["fr-ntt", "gl-batch", "fr-uneven", "serve", "ref-2p20", "poly-mul"])
     ap.add_argument("--log-n", type=int, default=24)
     ap.add_argument("--cpu-log-n", type=int, default=24, help="size of the CPU-baseline sample")
     ap.add_argument("--skip-cpu", action="store_true")
@@ -430,6 +487,8 @@ def main() -> int:
         line = run_fr_uneven(args, Z, ctx)
     elif args.workload == "ref-2p20":
         line = run_ref_2p20(args, Z, ctx)
+    elif args.workload == "poly-mul":
+        line = run_poly_mul(args, Z, ctx)
     else:
         line = run_serve(args, Z, ctx)
     print(json.dumps(line), flush=True)
