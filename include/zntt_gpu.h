@@ -70,7 +70,9 @@ enum zntt_status {
 
 enum zntt_field {
   ZNTT_FIELD_BN254_FR = 1,
-  ZNTT_FIELD_GOLDILOCKS = 2
+  ZNTT_FIELD_GOLDILOCKS = 2,
+  ZNTT_FIELD_BABYBEAR = 3,   /* p = 2^31 - 2^27 + 1, libs/field/src/predef/babybear.zig  */
+  ZNTT_FIELD_KOALABEAR = 4   /* p = 2^31 - 2^24 + 1, libs/field/src/predef/koalabear.zig */
 };
 
 /* Direction of a device transform (zntt_transform.direction). */

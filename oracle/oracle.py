@@ -24,11 +24,13 @@ import numpy as np
 FIELD_FR = 1
 FIELD_GL = 2
 FIELD_BB = 3
+FIELD_KB = 4
 
 FR_MODULUS = 0x30644E72E131A029B85045B68181585D2833E84879B9709143E1F593F0000001
 GL_MODULUS = 0xFFFFFFFF00000001
 BB_MODULUS = 0x78000001
-MODULUS = {FIELD_FR: FR_MODULUS, FIELD_GL: GL_MODULUS, FIELD_BB: BB_MODULUS}
+KB_MODULUS = 0x7F000001
+MODULUS = {FIELD_FR: FR_MODULUS, FIELD_GL: GL_MODULUS, FIELD_BB: BB_MODULUS, FIELD_KB: KB_MODULUS}
 
 OK = 0
 LOG_TOO_LARGE = 1

@@ -650,16 +650,18 @@ static uint64_t sf_draw(const sf_ctx *c, xo_t *x) {
 #define FIELD_FR 1
 #define FIELD_GL 2
 #define FIELD_BB 3
+#define FIELD_KB 4
 
 static int sf_for(int field, sf_ctx *c) {
   if (field == FIELD_GL) { *c = sf_make(0xFFFFFFFF00000001ull); return 1; }
   if (field == FIELD_BB) { *c = sf_make(0x78000001ull); return 1; }
+  if (field == FIELD_KB) { *c = sf_make(0x7F000001ull); return 1; }
   return 0;
 }
 
 int oracle_elem_size(int field) {
   if (field == FIELD_FR) return 32;
-  if (field == FIELD_GL || field == FIELD_BB) return 8;
+  if (field == FIELD_GL || field == FIELD_BB || field == FIELD_KB) return 8;
   return 0;
 }
 
@@ -898,7 +900,9 @@ int oracle_poly_eval(int field, const void *coeffs, size_t n, const void *point,
  * SplitMix64 finaliser, gamma its increment, key = mix64(mix64(seed) ^ id)
  * and ctr = i << 8 | attempt << 2 | word.  Fr: four little-endian words, the
  * top two bits cleared, accepted below r (first of up to 64 attempts) and
- * kept as the Montgomery words; Goldilocks: one word accepted below p. */
+ * kept as the Montgomery words; small fields: one word masked to the
+ * modulus' bit length (64 Goldilocks, 31 BabyBear / KoalaBear), accepted
+ * below p. */
 static uint64_t mix64(uint64_t z) {
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
   z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
@@ -922,13 +926,15 @@ int oracle_fill_counter(int field, uint64_t seed, uint64_t stream_id, void *out,
     }
     return OR_OK;
   }
-  if (field != FIELD_GL) return OR_INVALID_ARGUMENT;
+  sf_ctx c;
+  if (!sf_for(field, &c)) return OR_INVALID_ARGUMENT;
+  const uint64_t mask = c.bits >= 64 ? ~0ull : (((uint64_t)1 << c.bits) - 1);
   uint64_t *o = (uint64_t *)out;
   for (size_t i = 0; i < n; i++) {
     uint64_t w = 0;
     for (uint64_t att = 0; att < 64; att++) {
-      w = mix64(key + (((uint64_t)i << 8 | att << 2) + 1) * gamma);
-      if (w < 0xFFFFFFFF00000001ull) break;
+      w = mix64(key + (((uint64_t)i << 8 | att << 2) + 1) * gamma) & mask;
+      if (w < c.p) break;
     }
     o[i] = w;
   }

@@ -178,7 +178,8 @@ def _mix64(z: int) -> int:
 def counter_stored(p: int, seed: int, stream_id: int, n: int) -> list[int]:
     """Stored integers of elements 0..n-1 of stream (seed, stream_id): for Fr
     the 256-bit Montgomery word (four draws, top two bits cleared, first below
-    p), for Goldilocks the canonical value (one draw, first below p).  Draw
+    p), for the small fields the canonical value (one draw masked to p's bit
+    length, first below p).  Draw
     (i, attempt, word) is mix64(key + (i*256 + attempt*4 + word + 1) * gamma)
     with key = mix64(mix64(seed) ^ stream_id)."""
     gamma = 0x9E3779B97F4A7C15
@@ -192,6 +193,8 @@ def counter_stored(p: int, seed: int, stream_id: int, n: int) -> list[int]:
                 v |= _mix64((key + (i * 256 + att * 4 + w + 1) * gamma) & MASK64) << (64 * w)
             if p == FR:
                 v &= (1 << 254) - 1
+            else:
+                v &= (1 << p.bit_length()) - 1
             if v < p:
                 break
         out.append(v)

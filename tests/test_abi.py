@@ -13,6 +13,7 @@ import re
 import pytest
 
 import oracle as O
+import pyref
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "zntt_gpu.h")
@@ -54,6 +55,7 @@ def test_status_codes_match_oracle_and_reference_error_names(Z):
 def test_element_sizes(Z):
     assert Z.element_size(Z.FIELD_FR) == 32
     assert Z.element_size(Z.FIELD_GL) == 8
+    assert Z.element_size(Z.FIELD_BB) == 8 and Z.element_size(Z.FIELD_KB) == 8
     assert Z.element_size(99) == 0
 
 
@@ -69,6 +71,16 @@ def test_primitive_roots_host_side(Z):
         Z.primitive_root(Z.FIELD_FR, 29)
     with pytest.raises(Z.OrderTooLarge):
         Z.primitive_root(Z.FIELD_GL, 33)
+    # 31-bit fields: z^((p-1) >> l) with z the smallest non-residue, checked
+    # against the Python-int construction and for primitivity.
+    for F, two_adic in ((Z.FIELD_BB, 27), (Z.FIELD_KB, 24)):
+        p = Z.MODULUS[F]
+        for l in (0, 1, 12, two_adic):
+            r = int(Z.primitive_root(F, l)[0])
+            assert r == pyref.primitive_root(p, l)
+            assert pow(r, 1 << l, p) == 1 and (l == 0 or pow(r, 1 << (l - 1), p) == p - 1)
+        with pytest.raises(Z.OrderTooLarge):
+            Z.primitive_root(F, two_adic + 1)
 
 
 def test_context_without_gpu_reports_no_device(Z):
@@ -84,7 +96,7 @@ def test_library_generator_matches_oracle(Z):
     same stream as the oracle's and the Python-int restatement."""
     import numpy as np
     import pyref
-    for F in (Z.FIELD_FR, Z.FIELD_GL):
+    for F in (Z.FIELD_FR, Z.FIELD_GL, Z.FIELD_BB, Z.FIELD_KB):
         got = Z.fill_uniform(F, 3, 500)
         assert np.array_equal(got, O.gen_uniform(F, 3, 500))
     c = json.load(open(os.path.join(ROOT, "tests", "golden", "textbook_digests.json")))["cases"]["fr_2p16_seed0"]

@@ -61,9 +61,9 @@ def oracle_apply(field, x, log_n, direction, shift):
     return y
 
 
-def test_uneven_mixed_batch_matches_oracle(Z, W, ctx):
+@pytest.mark.parametrize("F", [O.FIELD_FR, O.FIELD_GL, O.FIELD_BB, O.FIELD_KB], ids=["fr", "gl", "bb", "kb"])
+def test_uneven_mixed_batch_matches_oracle(Z, W, ctx, F):
     import torch
-    F = Z.FIELD_FR
     spec = W.uneven_batch(count=24, lo=5, hi=14, seed=33)
     shift = O.from_int(F, 5)
     xs = [Z.fill_uniform(F, W.item_seed(33, i), 1 << L) for i, (L, d) in enumerate(spec)]
@@ -79,7 +79,7 @@ def test_uneven_mixed_batch_matches_oracle(Z, W, ctx):
             torch.cuda.synchronize()
             for t0, t1 in zip(before, ts):  # tuning runs on shadow buffers
                 assert torch.equal(t0, t1)
-            print("tuned shape", b.shape(), flush=True)
+            print("tuned shape", F, b.shape(), flush=True)
         b.execute(stream=torch.cuda.current_stream().cuda_stream)
         b.status(stream=torch.cuda.current_stream().cuda_stream)
         expect = [oracle_apply(F, e, L, d, shift) for e, (L, d) in zip(expect, spec)]

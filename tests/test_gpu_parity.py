@@ -20,8 +20,11 @@ import pyref
 pytestmark = pytest.mark.gpu
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-FIELDS = [O.FIELD_FR, O.FIELD_GL]
-FNAME = {O.FIELD_FR: "fr", O.FIELD_GL: "gl"}
+FIELDS = [O.FIELD_FR, O.FIELD_GL, O.FIELD_BB, O.FIELD_KB]
+IDS = ["fr", "gl", "bb", "kb"]
+FNAME = dict(zip(FIELDS, IDS))
+# Smallest generator of each multiplicative group: the coset shifts.
+GENERATOR = {O.FIELD_FR: 5, O.FIELD_GL: 7, O.FIELD_BB: 31, O.FIELD_KB: 3}
 
 
 def load(name):
@@ -54,7 +57,7 @@ def say(*a):
 # Device field arithmetic
 # ---------------------------------------------------------------------------
 
-@pytest.mark.parametrize("field", FIELDS, ids=["fr", "gl"])
+@pytest.mark.parametrize("field", FIELDS, ids=IDS)
 def test_device_field_ops_match_oracle(Z, ctx, field):
     say("field ops", FNAME[field])
     n = 4096
@@ -79,7 +82,7 @@ def test_device_field_ops_match_oracle(Z, ctx, field):
 SIZES = list(range(0, 15)) + [16, 17, 19]
 
 
-@pytest.mark.parametrize("field", FIELDS, ids=["fr", "gl"])
+@pytest.mark.parametrize("field", FIELDS, ids=IDS)
 @pytest.mark.parametrize("log_n", SIZES)
 def test_ntt_intt_match_oracle(Z, ctx, field, log_n):
     say("ntt/intt", FNAME[field], log_n)
@@ -101,17 +104,21 @@ def test_ntt_intt_match_oracle(Z, ctx, field, log_n):
 
 
 def test_reference_known_answers_on_device(Z, ctx):
-    """libs/ntt/src/root.zig:438-443 (the Goldilocks rows) through the GPU."""
+    """libs/ntt/src/root.zig:438-449 (the Goldilocks and BabyBear rows)
+    through the GPU."""
+    fid = {"goldilocks": Z.FIELD_GL, "babybear": Z.FIELD_BB}
+    seen = set()
     for c in load("ref_ntt_kat.json")["cases"]:
-        if c["field"] != "goldilocks":
-            continue
+        F = fid[c["field"]]
+        seen.add(F)
         xs = np.array([int(v, 16) for v in c["input"]], dtype=np.uint64)
         root = np.array([int(c["root"], 16)], dtype=np.uint64)
         data = xs.copy()
-        ctx.ntt(Z.FIELD_GL, data, c["log_n"], root)
-        assert [int(v) for v in data] == [int(v, 16) for v in c["expect"]]
-        ctx.intt(Z.FIELD_GL, data, c["log_n"], root)
+        ctx.ntt(F, data, c["log_n"], root)
+        assert [int(v) for v in data] == [int(v, 16) for v in c["expect"]], (c["field"], c["log_n"])
+        ctx.intt(F, data, c["log_n"], root)
         assert np.array_equal(data, xs)
+    assert seen == {Z.FIELD_GL, Z.FIELD_BB}
 
 
 def test_fr_dft_vectors_on_device(Z, ctx):
@@ -125,7 +132,7 @@ def test_fr_dft_vectors_on_device(Z, ctx):
         assert [O.value(F, data[i]) for i in range(data.shape[0])] == [int(v, 16) for v in c["dft"]]
 
 
-@pytest.mark.parametrize("field", FIELDS, ids=["fr", "gl"])
+@pytest.mark.parametrize("field", FIELDS, ids=IDS)
 def test_non_primitive_root_follows_the_dag(Z, ctx, field):
     """For a root that is not a primitive root the reference's DIT DAG is the
     only definition; the device must reproduce it exactly."""
@@ -143,7 +150,7 @@ def test_non_primitive_root_follows_the_dag(Z, ctx, field):
 # Twiddled variants
 # ---------------------------------------------------------------------------
 
-@pytest.mark.parametrize("field", FIELDS, ids=["fr", "gl"])
+@pytest.mark.parametrize("field", FIELDS, ids=IDS)
 @pytest.mark.parametrize("log_n", [1, 3, 10, 13, 16])
 def test_precompute_and_twiddled_transforms(Z, ctx, field, log_n):
     say("twiddled", FNAME[field], log_n)
@@ -167,7 +174,7 @@ def test_precompute_and_twiddled_transforms(Z, ctx, field, log_n):
     tw.free()
 
 
-@pytest.mark.parametrize("field", FIELDS, ids=["fr", "gl"])
+@pytest.mark.parametrize("field", FIELDS, ids=IDS)
 def test_foreign_non_power_tables(Z, ctx, field):
     """A caller's table that is not powers of anything (root.zig:148-212 uses
     it as given): forward and the -tw[h-j] inverse must match the oracle."""
@@ -238,7 +245,7 @@ def test_error_contracts(Z, ctx):
     tw.free()
 
 
-@pytest.mark.parametrize("field", FIELDS, ids=["fr", "gl"])
+@pytest.mark.parametrize("field", FIELDS, ids=IDS)
 def test_bit_reverse_matches_oracle(Z, ctx, field):
     for log_n in (0, 1, 2, 7, 13):
         x = O.gen_uniform(field, 60 + log_n, 1 << log_n)
@@ -253,9 +260,9 @@ def test_bit_reverse_matches_oracle(Z, ctx, field):
 # Coset and LDE
 # ---------------------------------------------------------------------------
 
-@pytest.mark.parametrize("field", FIELDS, ids=["fr", "gl"])
+@pytest.mark.parametrize("field", FIELDS, ids=IDS)
 def test_coset_and_lde_match_oracle_and_horner(Z, ctx, field):
-    shift = O.from_int(field, 5 if field == O.FIELD_FR else 7)
+    shift = O.from_int(field, GENERATOR[field])
     for log_n in (0, 1, 6, 12, 15):
         x = O.gen_uniform(field, 300 + log_n, 1 << log_n)
         root = O.primitive_root(field, log_n)[1]
@@ -264,7 +271,7 @@ def test_coset_and_lde_match_oracle_and_horner(Z, ctx, field):
         got = x.copy()
         ctx.coset_ntt(field, got, log_n, root, shift)
         assert np.array_equal(got, exp), log_n
-    fname = "fr" if field == O.FIELD_FR else "goldilocks"
+    fname = {O.FIELD_FR: "fr", O.FIELD_GL: "goldilocks"}.get(field)
     for c in load("dft_vectors.json")["lde"]:
         if c["field"] != fname:
             continue

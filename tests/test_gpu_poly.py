@@ -13,7 +13,8 @@ import pytest
 import oracle as O
 
 pytestmark = pytest.mark.gpu
-FIELDS = [O.FIELD_FR, O.FIELD_GL]
+FIELDS = [O.FIELD_FR, O.FIELD_GL, O.FIELD_BB, O.FIELD_KB]
+IDS = ["fr", "gl", "bb", "kb"]
 
 
 @pytest.fixture(scope="module")
@@ -29,7 +30,7 @@ def ctx(Z):
     c.close()
 
 
-@pytest.mark.parametrize("field", FIELDS, ids=["fr", "gl"])
+@pytest.mark.parametrize("field", FIELDS, ids=IDS)
 def test_poly_mul_matches_schoolbook(Z, ctx, field):
     sizes = [(1, 1), (1, 7), (2, 2), (3, 6), (16, 17), (33, 32), (64, 65), (100, 29), (255, 258), (513, 511)]
     for k, (la, lb) in enumerate(sizes):
@@ -41,7 +42,7 @@ def test_poly_mul_matches_schoolbook(Z, ctx, field):
         assert np.array_equal(got, want), (la, lb)
 
 
-@pytest.mark.parametrize("field", FIELDS, ids=["fr", "gl"])
+@pytest.mark.parametrize("field", FIELDS, ids=IDS)
 def test_poly_mul_edge_contracts(Z, ctx, field):
     a = O.gen_uniform(field, 1, 10)
     zero = O.empty(field, 5)
@@ -59,7 +60,7 @@ def test_poly_mul_edge_contracts(Z, ctx, field):
     assert np.array_equal(ctx.poly_mul(field, c1, c2), want)
 
 
-@pytest.mark.parametrize("field", FIELDS, ids=["fr", "gl"])
+@pytest.mark.parametrize("field", FIELDS, ids=IDS)
 def test_large_poly_mul_by_evaluation(Z, ctx, field):
     """2^18 x 2^18 coefficients (a 2^19-point product): (a*b)(z) = a(z) b(z)
     at three points, and the device entry point equals the host one."""

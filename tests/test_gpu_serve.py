@@ -39,7 +39,11 @@ def expected(field, seed, sid, k, b, shift):
     return out
 
 
-@pytest.mark.parametrize("field", [O.FIELD_FR, O.FIELD_GL])
+ALL = [O.FIELD_FR, O.FIELD_GL, O.FIELD_BB, O.FIELD_KB]
+IDS = ["fr", "gl", "bb", "kb"]
+
+
+@pytest.mark.parametrize("field", ALL, ids=IDS)
 def test_device_counter_stream_matches_oracle(Z, ctx, field):
     import torch
     words = 4 if field == O.FIELD_FR else 1
@@ -62,7 +66,7 @@ def mixed_requests(field, count, seed):
     return out, shifts
 
 
-@pytest.mark.parametrize("field", [O.FIELD_FR, O.FIELD_GL])
+@pytest.mark.parametrize("field", ALL, ids=IDS)
 def test_mixed_requests_equal_alone_and_oracle(Z, ctx, field):
     """40 requests of 2^3..2^12 coefficients, blowup 2..8, three shifts, all
     submitted at once (they share steps) and then one at a time: both runs

@@ -385,3 +385,25 @@ def test_poly_eval_matches_python_and_product_identity():
         lhs = O.value(F, O.poly_eval(F, r, O.from_int(F, z)))
         rhs = (sum(c * pow(z, i, p) for i, c in enumerate(ai)) * sum(c * pow(z, i, p) for i, c in enumerate(bi))) % p
         assert lhs == rhs
+
+
+def test_koalabear_small_field_by_definition():
+    """KoalaBear (libs/field/src/predef/koalabear.zig) has no reference vectors
+    on the ntt path; the generic small field is pinned by Python ints: field
+    ops, the primitive roots, and a DFT against its O(n^2) definition."""
+    F, p = O.FIELD_KB, O.KB_MODULUS
+    assert O.lib().oracle_elem_size(F) == 8
+    xs = [0, 1, 2, p - 1, 123456789, (p - 1) // 2]
+    for a in xs:
+        for b in xs:
+            ea, eb = O.from_int(F, a), O.from_int(F, b)
+            assert O.value(F, O.field_op(F, "mul", ea, eb)) == a * b % p
+            assert O.value(F, O.field_op(F, "add", ea, eb)) == (a + b) % p
+            assert O.value(F, O.field_op(F, "sub", ea, eb)) == (a - b) % p
+    for l in (1, 5, 24):
+        assert O.value(F, O.primitive_root(F, l)[1]) == pyref.primitive_root(p, l)
+    x = [int(v) % p for v in np.random.default_rng(3).integers(0, 1 << 62, 16)]
+    w = pyref.primitive_root(p, 4)
+    data = O.from_ints(F, x)
+    assert O.ntt(F, data, 4, O.from_int(F, w)) == O.OK
+    assert canon(F, data) == pyref.dft(x, w, p)

@@ -71,7 +71,8 @@ def run_to_completion(srv, max_iter=1000):
 
 
 def test_counter_stream_matches_python_restatement():
-    for field, p in ((O.FIELD_FR, pyref.FR), (O.FIELD_GL, pyref.GL)):
+    for field, p in ((O.FIELD_FR, pyref.FR), (O.FIELD_GL, pyref.GL), (O.FIELD_BB, O.BB_MODULUS),
+                     (O.FIELD_KB, O.KB_MODULUS)):
         for seed, sid in ((4, 0), (4, 17), (0, 1 << 40), (123456789, 3)):
             got = O.fill_counter(field, seed, sid, 40)
             want = pyref.counter_stored(p, seed, sid, 40)

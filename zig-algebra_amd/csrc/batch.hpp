@@ -43,12 +43,14 @@ hipError_t launch_batch_level(int field, const BatchXform *xf, const BatchEntry 
                               uint32_t nitems, uint32_t uniform_tiles, uint32_t *counter, int grid, int tile_log,
                               hipStream_t s);
 // One persistent launch over n transforms that each fit one tile of 2^K
-// elements (K = their log size, 9..11 for BN254 Fr, 9..12 for Goldilocks):
+// elements (K = their log size, 9..11 for BN254 Fr, 9..12 for the others):
 // transform ent[i].xf is item i.  `*counter` must be zero at launch.
 hipError_t launch_whole_fr(int K, const BatchXform *xf, const BatchEntry *ent, uint32_t n, uint32_t *counter,
                            hipStream_t s);
 hipError_t launch_whole_gl(int K, const BatchXform *xf, const BatchEntry *ent, uint32_t n, uint32_t *counter,
                            hipStream_t s);
+hipError_t launch_whole_f31(int field, int K, const BatchXform *xf, const BatchEntry *ent, uint32_t n,
+                            uint32_t *counter, hipStream_t s);
 inline int whole_max_log(int field) { return field == 1 ? 11 : 12; }
 constexpr int kWholeMinLog = 9;
 

@@ -247,11 +247,20 @@ hipError_t launch_batch_level(int field, const BatchXform *xf, const BatchEntry 
     const size_t lds = (size_t)9 * 4 << tile_log;
     hipLaunchKernelGGL((batch_kernel<FrT, 256, 2>), dim3(g), dim3(threads), lds, s, xf, ent, nent, nitems, uniform_tiles,
                        counter);
-  } else {
+  } else if (field == kGl) {
     if (tile_log > 12) return hipErrorInvalidValue;
     const size_t lds = (size_t)2 * 4 << tile_log;
     hipLaunchKernelGGL((batch_kernel<GlT, 512, 4>), dim3(g), dim3(threads), lds, s, xf, ent, nent, nitems, uniform_tiles,
                        counter);
+  } else {
+    if (tile_log > 12) return hipErrorInvalidValue;
+    const size_t lds = (size_t)4 << tile_log;
+    if (field == kBb)
+      hipLaunchKernelGGL((batch_kernel<BbT, 512, 4>), dim3(g), dim3(threads), lds, s, xf, ent, nent, nitems,
+                         uniform_tiles, counter);
+    else
+      hipLaunchKernelGGL((batch_kernel<KbT, 512, 4>), dim3(g), dim3(threads), lds, s, xf, ent, nent, nitems,
+                         uniform_tiles, counter);
   }
   return hipGetLastError();
 }
@@ -261,9 +270,15 @@ hipError_t configure_batch_lds() {
   if (done) return hipSuccess;
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&batch_kernel<FrT, 256, 2>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, 9 * 4 * (1 << 11));
-  if (e != hipSuccess) return e;
-  e = hipFuncSetAttribute(reinterpret_cast<const void *>(&batch_kernel<GlT, 512, 4>),
-                          hipFuncAttributeMaxDynamicSharedMemorySize, 2 * 4 * (1 << 12));
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute(reinterpret_cast<const void *>(&batch_kernel<GlT, 512, 4>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, 2 * 4 * (1 << 12));
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute(reinterpret_cast<const void *>(&batch_kernel<BbT, 512, 4>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, 4 * (1 << 12));
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute(reinterpret_cast<const void *>(&batch_kernel<KbT, 512, 4>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, 4 * (1 << 12));
   if (e == hipSuccess) done = true;
   return e;
 }
@@ -275,8 +290,9 @@ hipError_t configure_batch_lds() {
 // mix64 is the SplitMix64 finaliser and gamma its increment.  Fr takes four
 // words (little-endian), clears the top two bits and accepts a value below r
 // (the first of up to 64 attempts; the value is stored as the Montgomery
-// words, so the represented element is uniform too); Goldilocks takes one
-// word and accepts it below p.
+// words, so the represented element is uniform too); the small fields take
+// one word, masked to the modulus' bit length (64 for Goldilocks, 31 for
+// BabyBear and KoalaBear), and accept it below p.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ uint64_t mix64(uint64_t z) {
   z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
@@ -312,12 +328,13 @@ __global__ void fill_counter_fr_kernel(uint32_t *out, size_t n, uint64_t key) {
   }
 }
 
-__global__ void fill_counter_gl_kernel(uint32_t *out, size_t n, uint64_t key) {
+// Small fields: one word, masked to the modulus' bit length, accepted below p.
+__global__ void fill_counter_small_kernel(uint32_t *out, size_t n, uint64_t key, uint64_t p, uint64_t mask) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
     uint64_t w = 0;
     for (uint64_t att = 0; att < 64; att++) {
-      w = mix64(key + (((uint64_t)i << 8 | att << 2) + 1) * kGamma);
-      if (w < 0xffffffff00000001ull) break;
+      w = mix64(key + (((uint64_t)i << 8 | att << 2) + 1) * kGamma) & mask;
+      if (w < p) break;
     }
     reinterpret_cast<uint2 *>(out)[i] = make_uint2((uint32_t)w, (uint32_t)(w >> 32));
   }
@@ -337,10 +354,13 @@ hipError_t launch_fill_counter(int field, uint32_t *out, size_t n, uint64_t seed
   if (n == 0) return hipSuccess;
   const uint64_t key = counter_key(seed, stream_id);
   const size_t blocks = std::min<size_t>((n + 255) / 256, 8192);
-  if (field == kFr)
+  if (field == kFr) {
     hipLaunchKernelGGL(fill_counter_fr_kernel, dim3((unsigned)blocks), dim3(256), 0, s, out, n, key);
-  else
-    hipLaunchKernelGGL(fill_counter_gl_kernel, dim3((unsigned)blocks), dim3(256), 0, s, out, n, key);
+  } else {
+    const uint64_t p = field == kGl ? 0xffffffff00000001ull : (field == kBb ? kBbP : kKbP);
+    const uint64_t mask = field == kGl ? ~0ull : 0x7fffffffull;
+    hipLaunchKernelGGL(fill_counter_small_kernel, dim3((unsigned)blocks), dim3(256), 0, s, out, n, key, p, mask);
+  }
   return hipGetLastError();
 }
 

@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include "field_fr.hpp"
+#include "field_f31.hpp"
 #include "field_gl.hpp"
 
 namespace zntt {
@@ -152,6 +153,63 @@ struct GlT {
 
 __device__ __forceinline__ uint32_t brev_bits(uint32_t x, int bits) {
   return bits == 0 ? 0u : (__brev(x) >> (32 - bits));
+}
+
+// 31-bit fields: one u32 per element in registers, LDS and scratch (dev
+// layout 4 B); the caller's buffer holds u64 (user layout 8 B).
+template <uint32_t P, uint32_t NPRIME>
+struct F31T {
+  using A = F31Arith<P, NPRIME>;
+  using E = uint32_t;
+  static constexpr int W = 1;
+  static constexpr int UW = 2;
+  static constexpr uint32_t kP = P;
+  __device__ static E load_user(const uint32_t *base, size_t i) { return base[2 * i]; }
+  __device__ static void store_user(uint32_t *base, size_t i, E x) {
+    reinterpret_cast<uint2 *>(base)[i] = make_uint2(x, 0u);
+  }
+  __device__ static E load_dev(const uint32_t *base, size_t i) { return base[i]; }
+  __device__ static void store_dev(uint32_t *base, size_t i, const E &x) { base[i] = x; }
+  __device__ static E lds_get(const uint32_t *lds, int, int i) { return lds[i]; }
+  __device__ static void lds_put(uint32_t *lds, int, int i, const E &x) { lds[i] = x; }
+  __device__ static E from_words(const uint32_t *w) { return w[0]; }
+  __device__ static E zero() { return 0; }
+  __device__ static E mul(const E &a, const E &b) { return A::mont(a, b); }
+  __device__ static void mul2(E &x0, const E &b0, E &x1, const E &b1) {
+    x0 = A::mont(x0, b0);
+    x1 = A::mont(x1, b1);
+  }
+  template <bool NORM = true>
+  __device__ static void bfly(E &u, E &v, const E &w) {
+    A::butterfly(u, v, w);
+  }
+  template <bool NORM = true>
+  __device__ static void bfly2(E &u0, E &v0, const E &w0, E &u1, E &v1, const E &w1) {
+    A::butterfly(u0, v0, w0);
+    A::butterfly(u1, v1, w1);
+  }
+  template <bool NORM = true>
+  __device__ static void bfly_one(E &u, E &v) {
+    A::butterfly_one(u, v);
+  }
+  static constexpr bool one_ok(int) { return true; }
+  __device__ static E canon(E x) { return A::canon(x); }
+  __device__ static E neg(const E &x) { return x == 0 ? 0 : P - x; }
+};
+
+using BbT = F31T<kBbP, kBbNPrime>;
+using KbT = F31T<kKbP, kKbNPrime>;
+
+// Calls fn(F{}) with the traits type of a field id (1 BN254 Fr, 2 Goldilocks,
+// 3 BabyBear, 4 KoalaBear).
+template <class Fn>
+__host__ auto with_field(int field, Fn &&fn) {
+  switch (field) {
+    case 1: return fn(FrT{});
+    case 2: return fn(GlT{});
+    case 3: return fn(BbT{});
+    default: return fn(KbT{});
+  }
 }
 
 }  // namespace zntt

@@ -181,6 +181,31 @@ struct Fr {
 // ---------------------------------------------------------------------------
 // Goldilocks p = 2^64 - 2^32 + 1, canonical u64 (the reference's storage form).
 // ---------------------------------------------------------------------------
+// 31-bit prime fields (BabyBear, KoalaBear): canonical values; the device
+// keeps constants in Montgomery form c * 2^32 mod p.
+struct F31 {
+  uint64_t p;
+  int two_adicity;
+  uint64_t qnr;  // smallest quadratic non-residue (roots.zig:149-157)
+  uint64_t mul(uint64_t a, uint64_t b) const { return a * b % p; }
+  uint64_t pow(uint64_t a, uint64_t e) const {
+    uint64_t r = 1, b = a % p;
+    while (e) {
+      if (e & 1) r = mul(r, b);
+      b = mul(b, b);
+      e >>= 1;
+    }
+    return r;
+  }
+  uint64_t inv(uint64_t a) const { return a % p == 0 ? 0 : pow(a, p - 2); }
+  // z^(odd << (two_adicity - log_size)) = z^((p - 1) >> log_size)
+  uint64_t primitive_root(int log_size) const { return pow(qnr, (p - 1) >> log_size); }
+  uint32_t to_mont(uint64_t a) const { return (uint32_t)(((a % p) << 32) % p); }
+};
+
+constexpr F31 kBabyBear = {0x78000001ull, 27, 11};
+constexpr F31 kKoalaBear = {0x7f000001ull, 24, 3};
+
 struct Gl {
   static constexpr uint64_t P = 0xFFFFFFFF00000001ull;
   static uint64_t mul(uint64_t a, uint64_t b) { return (uint64_t)(((u128)a * b) % P); }

@@ -6,13 +6,13 @@
 
 namespace zntt {
 
-enum FieldId { kFr = 1, kGl = 2 };
+enum FieldId { kFr = 1, kGl = 2, kBb = 3, kKb = 4 };
 
 // Words (u32) per element in the kernels' two layouts.
 //  user: the reference's storage layout (Fr 8, Goldilocks 2)
 //  dev:  scratch buffers and twiddle tables (Fr 9 radix-2^29 limbs, Goldilocks 2)
 inline int user_words(int field) { return field == kFr ? 8 : 2; }
-inline int dev_words(int field) { return field == kFr ? 9 : 2; }
+inline int dev_words(int field) { return field == kFr ? 9 : (field == kGl ? 2 : 1); }
 
 // One pass covers DIT stages s0 .. s0+K-1 (1-based) of a 2^L-point transform.
 struct PassArgs {
@@ -54,6 +54,8 @@ hipError_t launch_transform_fr(int L, uint32_t *data, uint32_t *scratch, const u
                                const uint32_t *scale_dev, bool tw_one, hipStream_t stream);
 hipError_t launch_transform_gl(int L, uint32_t *data, uint32_t *scratch, const uint32_t *tw, const uint32_t *coset,
                                const uint32_t *scale_dev, bool tw_one, hipStream_t stream);
+hipError_t launch_transform_f31(int field, int L, uint32_t *data, uint32_t *scratch, const uint32_t *tw,
+                                const uint32_t *coset, const uint32_t *scale_dev, bool tw_one, hipStream_t stream);
 
 // Twiddle table builders (dev layout).  `flat` holds 2^L entries: stages
 // 1..L at offsets 2^(s-1)-1 and a trailing one (index 2^L - 1).

@@ -33,7 +33,7 @@
 
 namespace zntt {
 
-static int tile_log_for(int field) { return field == kFr ? 11 : 12; }
+static int tile_log_for(int field) { return field == kFr ? 11 : 12; }  // LDS: 72 KiB Fr, 32 / 16 KiB others
 
 PassPlan plan_passes(int field, int L) {
   const int tl = tile_log_for(field);
@@ -67,7 +67,8 @@ hipError_t launch_transform(int field, int L, uint32_t *data, uint32_t *scratch,
                             const uint32_t *coset, const uint32_t *scale_dev, bool tw_one, hipStream_t stream) {
   if (L == 0) return hipSuccess;
   if (field == kFr) return launch_transform_fr(L, data, scratch, tw, coset, scale_dev, tw_one, stream);
-  return launch_transform_gl(L, data, scratch, tw, coset, scale_dev, tw_one, stream);
+  if (field == kGl) return launch_transform_gl(L, data, scratch, tw, coset, scale_dev, tw_one, stream);
+  return launch_transform_f31(field, L, data, scratch, tw, coset, scale_dev, tw_one, stream);
 }
 
 // ---------------------------------------------------------------------------
@@ -135,18 +136,18 @@ static hipError_t build_table_t(int L, uint32_t *flat, const uint32_t *pows, con
 hipError_t launch_build_table(int field, int L, uint32_t *flat, const uint32_t *pows, const uint32_t *one_dev,
                               hipStream_t stream) {
   if (L == 0) return hipMemcpyAsync(flat, one_dev, 4 * dev_words(field), hipMemcpyDeviceToDevice, stream);
-  if (field == kFr) return build_table_t<FrT>(L, flat, pows, one_dev, stream);
-  return build_table_t<GlT>(L, flat, pows, one_dev, stream);
+  return with_field(field, [&](auto f) { return build_table_t<decltype(f)>(L, flat, pows, one_dev, stream); });
 }
 
 hipError_t launch_powers(int field, size_t count, uint32_t *out, const uint32_t *pows, int npows,
                          const uint32_t *one_dev, hipStream_t stream) {
   const size_t threads = (count + kRun - 1) / kRun;
   const unsigned blocks = (unsigned)((threads + 255) / 256);
-  if (field == kFr)
-    hipLaunchKernelGGL(powers_kernel<FrT>, dim3(blocks), dim3(256), 0, stream, out, count, pows, npows, one_dev);
-  else
-    hipLaunchKernelGGL(powers_kernel<GlT>, dim3(blocks), dim3(256), 0, stream, out, count, pows, npows, one_dev);
+  with_field(field, [&](auto f) {
+    hipLaunchKernelGGL(powers_kernel<decltype(f)>, dim3(blocks), dim3(256), 0, stream, out, count, pows, npows,
+                       one_dev);
+    return 0;
+  });
   return hipGetLastError();
 }
 
@@ -173,10 +174,39 @@ __global__ void fr_to_user_kernel(uint32_t *user, const uint32_t *dev, size_t co
   FrT::store_user(user, i, x);
 }
 
+// 31-bit fields: dev = mont(x, 2^64 mod p) = x * 2^32; user = mont(dev, 1).
+template <class F>
+__global__ void f31_from_user_kernel(uint32_t *dev, const uint32_t *user, size_t count, uint32_t r2) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) dev[i] = F::A::mont(F::load_user(user, i), r2);
+}
+
+template <class F>
+__global__ void f31_to_user_kernel(uint32_t *user, const uint32_t *dev, size_t count) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) F::store_user(user, i, F::A::mont(dev[i], 1u));
+}
+
+static uint32_t f31_r2(int field) {
+  const uint64_t p = field == kBb ? kBbP : kKbP;
+  const uint64_t r = ((uint64_t)1 << 32) % p;
+  return (uint32_t)(r * r % p);
+}
+
 hipError_t launch_table_from_user(int field, size_t count, uint32_t *dev, const uint32_t *user,
                                   hipStream_t stream) {
   if (count == 0) return hipSuccess;
   if (field == kGl) return hipMemcpyAsync(dev, user, count * 8, hipMemcpyDeviceToDevice, stream);
+  if (field == kBb || field == kKb) {
+    const uint32_t r2 = f31_r2(field);
+    if (field == kBb)
+      hipLaunchKernelGGL(f31_from_user_kernel<BbT>, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, stream,
+                         dev, user, count, r2);
+    else
+      hipLaunchKernelGGL(f31_from_user_kernel<KbT>, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, stream,
+                         dev, user, count, r2);
+    return hipGetLastError();
+  }
   hipLaunchKernelGGL(fr_from_user_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, stream, dev, user,
                      count);
   return hipGetLastError();
@@ -185,6 +215,15 @@ hipError_t launch_table_from_user(int field, size_t count, uint32_t *dev, const 
 hipError_t launch_table_to_user(int field, size_t count, uint32_t *user, const uint32_t *dev, hipStream_t stream) {
   if (count == 0) return hipSuccess;
   if (field == kGl) return hipMemcpyAsync(user, dev, count * 8, hipMemcpyDeviceToDevice, stream);
+  if (field == kBb || field == kKb) {
+    if (field == kBb)
+      hipLaunchKernelGGL(f31_to_user_kernel<BbT>, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, stream, user,
+                         dev, count);
+    else
+      hipLaunchKernelGGL(f31_to_user_kernel<KbT>, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, stream, user,
+                         dev, count);
+    return hipGetLastError();
+  }
   hipLaunchKernelGGL(fr_to_user_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, stream, user, dev,
                      count);
   return hipGetLastError();
@@ -211,10 +250,11 @@ hipError_t launch_inverse_table(int field, int L, uint32_t *inv, const uint32_t 
   const size_t total = ((size_t)1 << L);
   const uint32_t *one_dev = fwd + (total - 1) * dev_words(field);
   const unsigned blocks = (unsigned)((total + 255) / 256);
-  if (field == kFr)
-    hipLaunchKernelGGL(inverse_table_kernel<FrT>, dim3(blocks), dim3(256), 0, stream, inv, fwd, L, one_dev);
-  else
-    hipLaunchKernelGGL(inverse_table_kernel<GlT>, dim3(blocks), dim3(256), 0, stream, inv, fwd, L, one_dev);
+  with_field(field, [&](auto f) {
+    hipLaunchKernelGGL(inverse_table_kernel<decltype(f)>, dim3(blocks), dim3(256), 0, stream, inv, fwd, L,
+                       one_dev);
+    return 0;
+  });
   return hipGetLastError();
 }
 
@@ -245,10 +285,10 @@ __global__ void pointwise_mul_kernel(uint32_t *a, const uint32_t *b, size_t n) {
 hipError_t launch_pointwise_mul(int field, uint32_t *a, const uint32_t *b, size_t n, hipStream_t stream) {
   if (n == 0) return hipSuccess;
   const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 65536);
-  if (field == kFr)
-    hipLaunchKernelGGL(pointwise_mul_kernel<FrT>, dim3(blocks), dim3(256), 0, stream, a, b, n);
-  else
-    hipLaunchKernelGGL(pointwise_mul_kernel<GlT>, dim3(blocks), dim3(256), 0, stream, a, b, n);
+  with_field(field, [&](auto f) {
+    hipLaunchKernelGGL(pointwise_mul_kernel<decltype(f)>, dim3(blocks), dim3(256), 0, stream, a, b, n);
+    return 0;
+  });
   return hipGetLastError();
 }
 
@@ -289,6 +329,16 @@ __global__ void fr_op_kernel(int op, size_t n, const uint32_t *a, const uint32_t
   FrT::store_user(out, i, r);
 }
 
+// 31-bit ops on user-layout operands; mul = mont(mont(x, y), 2^64 mod p).
+template <class F>
+__global__ void f31_op_kernel(int op, size_t n, const uint32_t *a, const uint32_t *b, uint32_t *out, uint32_t r2) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t x = F::load_user(a, i), y = F::load_user(b, i);
+  const uint32_t r = op == 0 ? F::A::add(x, y) : (op == 1 ? F::A::sub(x, y) : F::A::mont(F::A::mont(x, y), r2));
+  F::store_user(out, i, r);
+}
+
 __global__ void gl_op_kernel(int op, size_t n, const uint64_t *a, const uint64_t *b, uint64_t *out) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -302,9 +352,13 @@ hipError_t launch_field_op(int field, int op, size_t n, const uint32_t *a, const
   const unsigned blocks = (unsigned)((n + 255) / 256);
   if (field == kFr)
     hipLaunchKernelGGL(fr_op_kernel, dim3(blocks), dim3(256), 0, stream, op, n, a, b, out);
-  else
+  else if (field == kGl)
     hipLaunchKernelGGL(gl_op_kernel, dim3(blocks), dim3(256), 0, stream, op, n, (const uint64_t *)a,
                        (const uint64_t *)b, (uint64_t *)out);
+  else if (field == kBb)
+    hipLaunchKernelGGL(f31_op_kernel<BbT>, dim3(blocks), dim3(256), 0, stream, op, n, a, b, out, f31_r2(field));
+  else
+    hipLaunchKernelGGL(f31_op_kernel<KbT>, dim3(blocks), dim3(256), 0, stream, op, n, a, b, out, f31_r2(field));
   return hipGetLastError();
 }
 

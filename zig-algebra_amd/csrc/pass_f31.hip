@@ -1,0 +1,32 @@
+// BabyBear and KoalaBear instantiations of the pass kernels.
+#include "pass_kernels.hpp"
+
+namespace zntt {
+
+template <class F>
+static hipError_t launch_f31_t(int field, int L, uint32_t *data, uint32_t *scratch, const uint32_t *tw,
+                               const uint32_t *coset, const uint32_t *scale_dev, bool tw_one, hipStream_t stream) {
+  static bool configured = false;
+  if (!configured) {
+    hipError_t e = configure_pass_lds<F>(4 * (1 << 12));
+    if (e != hipSuccess) return e;
+    configured = true;
+  }
+  return run_passes_t<F>(field, L, data, scratch, tw, coset, scale_dev, tw_one, stream);
+}
+
+hipError_t launch_transform_f31(int field, int L, uint32_t *data, uint32_t *scratch, const uint32_t *tw,
+                                const uint32_t *coset, const uint32_t *scale_dev, bool tw_one, hipStream_t stream) {
+  if (field == kBb) return launch_f31_t<BbT>(field, L, data, scratch, tw, coset, scale_dev, tw_one, stream);
+  if (field == kKb) return launch_f31_t<KbT>(field, L, data, scratch, tw, coset, scale_dev, tw_one, stream);
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_whole_f31(int field, int K, const BatchXform *xf, const BatchEntry *ent, uint32_t n,
+                            uint32_t *counter, hipStream_t s) {
+  if (field == kBb) return launch_whole_t<BbT>(K, xf, ent, n, counter, s);
+  if (field == kKb) return launch_whole_t<KbT>(K, xf, ent, n, counter, s);
+  return hipErrorInvalidValue;
+}
+
+}  // namespace zntt

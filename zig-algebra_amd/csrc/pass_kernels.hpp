@@ -413,7 +413,7 @@ static hipError_t launch_whole_k(const BatchXform *xf, const BatchEntry *ent, ui
 }
 
 // Transforms of 2^9 up to the field's largest register-round tile (BN254 Fr
-// 2^11, Goldilocks 2^12).
+// 2^11, the other fields 2^12).
 template <class F>
 static hipError_t launch_whole_t(int K, const BatchXform *xf, const BatchEntry *ent, uint32_t n, uint32_t *counter,
                                  hipStream_t s) {
@@ -423,7 +423,7 @@ static hipError_t launch_whole_t(int K, const BatchXform *xf, const BatchEntry *
     case 10: return launch_whole_k<F, 10>(xf, ent, n, counter, s);
     case 11: return launch_whole_k<F, 11>(xf, ent, n, counter, s);
     case 12:
-      if constexpr (F::W == 2) return launch_whole_k<F, 12>(xf, ent, n, counter, s);
+      if constexpr (F::W <= 2) return launch_whole_k<F, 12>(xf, ent, n, counter, s);
       return hipErrorInvalidValue;
     default: return hipErrorInvalidValue;
   }

@@ -22,7 +22,11 @@ using namespace zntt_detail;
 namespace zntt_detail {
 
 
-bool field_ok(int field) { return field == ZNTT_FIELD_BN254_FR || field == ZNTT_FIELD_GOLDILOCKS; }
+bool field_ok(int field) { return field >= ZNTT_FIELD_BN254_FR && field <= ZNTT_FIELD_KOALABEAR; }
+const zntt::host::F31 *f31(int field) {
+  return field == ZNTT_FIELD_BABYBEAR ? &zntt::host::kBabyBear
+                                      : (field == ZNTT_FIELD_KOALABEAR ? &zntt::host::kKoalaBear : nullptr);
+}
 size_t user_bytes(int field) { return field == ZNTT_FIELD_BN254_FR ? 32 : 8; }
 size_t dev_bytes(int field) { return 4 * (size_t)zntt::dev_words(field); }
 
@@ -53,27 +57,38 @@ Elem from_u64e(uint64_t v) {
 
 Elem f_mul(int field, const Elem &a, const Elem &b) {
   if (field == ZNTT_FIELD_BN254_FR) return from_u256(zntt::host::Fr::mont(to_u256(a), to_u256(b)));
+  if (const auto *f = f31(field)) return from_u64e(f->mul(to_u64(a), to_u64(b)));
   return from_u64e(zntt::host::Gl::mul(to_u64(a), to_u64(b)));
 }
 Elem f_inv(int field, const Elem &a) {
   if (field == ZNTT_FIELD_BN254_FR) return from_u256(zntt::host::Fr::inv(to_u256(a)));
+  if (const auto *f = f31(field)) return from_u64e(f->inv(to_u64(a)));
   return from_u64e(zntt::host::Gl::inv(to_u64(a)));
 }
 Elem f_from_u64(int field, uint64_t v) {
   if (field == ZNTT_FIELD_BN254_FR) return from_u256(zntt::host::Fr::from_u64(v));
+  if (const auto *f = f31(field)) return from_u64e(v % f->p);
   return from_u64e(v % zntt::host::Gl::P);
 }
 Elem f_one(int field) { return f_from_u64(field, 1); }
 Elem f_primitive_root(int field, int log_size) {
   if (field == ZNTT_FIELD_BN254_FR) return from_u256(zntt::host::Fr::primitive_root(log_size));
+  if (const auto *f = f31(field)) return from_u64e(f->primitive_root(log_size));
   return from_u64e(zntt::host::Gl::primitive_root(log_size));
 }
-int two_adicity(int field) { return field == ZNTT_FIELD_BN254_FR ? 28 : 32; }
-// Device constant form (Fr: 29-bit limbs of c * 2^261; Goldilocks: the value).
+int two_adicity(int field) {
+  if (field == ZNTT_FIELD_BN254_FR) return 28;
+  if (const auto *f = f31(field)) return f->two_adicity;
+  return 32;
+}
+// Device constant form (Fr: 29-bit limbs of c * 2^261; Goldilocks: the value;
+// 31-bit fields: c * 2^32 mod p).
 std::array<uint32_t, 9> f_to_dev(int field, const Elem &a) {
   std::array<uint32_t, 9> d = {};
   if (field == ZNTT_FIELD_BN254_FR) {
     zntt::host::Fr::to_dev(to_u256(a), d.data());
+  } else if (const auto *f = f31(field)) {
+    d[0] = f->to_mont(to_u64(a));
   } else {
     d[0] = a.w[0];
     d[1] = a.w[1];
@@ -683,7 +698,12 @@ int poly_mul_device(zntt_ctx *c, int field, const void *a, size_t na, const void
   const Elem root = f_primitive_root(field, L);
   std::array<uint32_t, 9> scale;
   Elem sc = f_inv(field, f_from_u64(field, n));
+  // The pointwise Montgomery product of two data values carries a factor
+  // 2^-5 for BN254 Fr (data in 2^256 form, R' = 2^261) and 2^-32 for the
+  // 31-bit fields (plain data, R = 2^32); Goldilocks multiplies exactly.
   if (field == ZNTT_FIELD_BN254_FR) sc = f_mul(field, sc, f_from_u64(field, 32));
+  if (field == ZNTT_FIELD_BABYBEAR || field == ZNTT_FIELD_KOALABEAR)
+    sc = f_mul(field, sc, f_from_u64(field, (uint64_t)1 << 32));
   scale = f_to_dev(field, sc);
   {
     TwiddleEntry *fw = nullptr, *inv = nullptr;
@@ -861,10 +881,15 @@ int zntt_fill_uniform(int field, uint64_t seed, void *out, size_t n) {
       if (zntt::host::Fr::lt(v, zntt::host::Fr::P)) o[i++] = zntt::host::Fr::from_canonical(v);
     }
   } else {
+    // ceil(bits / 8) little-endian bytes of each draw, rejected at or above
+    // p (field.zig:224-232): 8 bytes for Goldilocks, 4 for the 31-bit fields.
+    const auto *f = f31(field);
+    const uint64_t p = f ? f->p : zntt::host::Gl::P;
+    const uint64_t mask = f ? 0xFFFFFFFFull : ~0ull;
     uint64_t *o = (uint64_t *)out;
     for (size_t i = 0; i < n;) {
-      const uint64_t v = next();
-      if (v < zntt::host::Gl::P) o[i++] = v;
+      const uint64_t v = next() & mask;
+      if (v < p) o[i++] = v;
     }
   }
   return ZNTT_OK;

@@ -167,8 +167,11 @@ hipError_t enqueue(zntt_batch *b, hipStream_t s) {
                                    b->d_counters + i, b->grid, b->shape.tile_log, s);
     else if (b->field == ZNTT_FIELD_BN254_FR)
       e = zntt::launch_whole_fr(l.whole_log, b->d_xf, b->d_ent + l.ent_off, l.nent, b->d_counters + i, s);
-    else
+    else if (b->field == ZNTT_FIELD_GOLDILOCKS)
       e = zntt::launch_whole_gl(l.whole_log, b->d_xf, b->d_ent + l.ent_off, l.nent, b->d_counters + i, s);
+    else
+      e = zntt::launch_whole_f31(b->field, l.whole_log, b->d_xf, b->d_ent + l.ent_off, l.nent, b->d_counters + i,
+                                 s);
   }
   return e;
 }

@@ -591,6 +591,17 @@ struct HipDevice : Device {
   }
 };
 
+// Smallest generator of the multiplicative group: the coset shift prepared
+// at creation (5 BN254 Fr, 7 Goldilocks, 31 BabyBear, 3 KoalaBear).
+uint64_t default_generator(int field) {
+  switch (field) {
+    case ZNTT_FIELD_BN254_FR: return 5;
+    case ZNTT_FIELD_GOLDILOCKS: return 7;
+    case ZNTT_FIELD_BABYBEAR: return 31;
+    default: return 3;
+  }
+}
+
 int check_request(const zntt_server *s, const zntt_lde_request *r) {
   if (!r || !r->shift) return ZNTT_INVALID_ARGUMENT;
   if (r->log_n >= 64 || r->log_blowup >= 64 || r->log_n + r->log_blowup >= 64) return ZNTT_LOG_TOO_LARGE;
@@ -617,7 +628,7 @@ int zntt_server_create(zntt_ctx *ctx, const zntt_server_config *cfg, zntt_server
   s->cfg = *cfg;
   if (!s->cfg.hbm_budget) s->cfg.hbm_budget = kDefaultBudget;
   if (!s->cfg.max_active) s->cfg.max_active = kDefaultActive;
-  if (cfg->field == ZNTT_FIELD_GOLDILOCKS) {
+  if (cfg->field != ZNTT_FIELD_BN254_FR) {
     s->tile_log = 12;
     s->kmax = 10;
   }
@@ -637,7 +648,7 @@ int zntt_server_create(zntt_ctx *ctx, const zntt_server_config *cfg, zntt_server
       st = power_table(ctx, cfg->field, f_primitive_root(cfg->field, cfg->max_log_n), cfg->max_log_n, ctx->stream,
                        &t);
       if (st) return st;
-      const Elem gen = f_from_u64(cfg->field, cfg->field == ZNTT_FIELD_BN254_FR ? 5 : 7);
+      const Elem gen = f_from_u64(cfg->field, default_generator(cfg->field));
       const uint32_t *cos = nullptr;
       st = coset_table(ctx, cfg->field, gen, (size_t)1 << (cfg->max_log_n - 3), ctx->stream, &cos);
       if (st) return st;

@@ -19,6 +19,8 @@ import numpy as np
 
 FIELD_FR = 1
 FIELD_GL = 2
+FIELD_BB = 3  # BabyBear, p = 2^31 - 2^27 + 1
+FIELD_KB = 4  # KoalaBear, p = 2^31 - 2^24 + 1
 
 FORWARD = 0
 INVERSE = 1
@@ -242,6 +244,9 @@ def fill_uniform(field: int, seed: int, n: int) -> np.ndarray:
 
 FR_MODULUS = 0x30644E72E131A029B85045B68181585D2833E84879B9709143E1F593F0000001
 GL_MODULUS = 0xFFFFFFFF00000001
+BB_MODULUS = 0x78000001
+KB_MODULUS = 0x7F000001
+MODULUS = {FIELD_FR: FR_MODULUS, FIELD_GL: GL_MODULUS, FIELD_BB: BB_MODULUS, FIELD_KB: KB_MODULUS}
 
 
 def from_int(field: int, value: int) -> np.ndarray:
@@ -249,7 +254,7 @@ def from_int(field: int, value: int) -> np.ndarray:
     if field == FIELD_FR:
         m = ((value % FR_MODULUS) << 256) % FR_MODULUS
         return np.array([(m >> (64 * i)) & ((1 << 64) - 1) for i in range(4)], dtype=np.uint64)
-    return np.array([value % GL_MODULUS], dtype=np.uint64)
+    return np.array([value % MODULUS[field]], dtype=np.uint64)
 
 
 def primitive_root(field: int, log_size: int) -> np.ndarray:
