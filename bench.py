@@ -9,6 +9,7 @@ iteration (libs/pairing/src/bench.zig:16-24, 107-111).
 
 Other configurations print their own JSON line when asked for:
   --workload gl-batch   4096 Goldilocks transforms of 2^12, one persistent launch
+  --workload bb-batch / kb-batch  the same batch over BabyBear / KoalaBear
   --workload fr-uneven  512 BN254 Fr transforms, 2^10..2^20, forward / inverse /
                         coset-forward, one persistent launch vs one call each
   --workload ref-2p20   the reference's own NTT benchmark: nttWithTwiddles over
@@ -144,6 +145,62 @@ def run_fr_ntt(args, Z, ctx):
                       "field": "bn254_fr", "log_n": log_n, "elements": n},
                      "synthetic: uniform BN254 Fr in [0, r), seed 1 (SplitMix64/xoshiro256++)", "u32")
     line.update({"roofline": roofline, "cpu_baseline": cpu, "extras": extras})
+    return line
+
+
+def run_small_batch(args, Z, ctx, field):
+    """4096 forward transforms of 2^12 over a 31-bit field (BabyBear or
+    KoalaBear): the Goldilocks batch shape with one third of its arithmetic."""
+    import numpy as np
+    import torch
+    import oracle as O
+    count, L = 4096, 12
+    name = {Z.FIELD_BB: "BabyBear", Z.FIELD_KB: "KoalaBear"}[field]
+    n = count << L
+    x = Z.fill_uniform(field, 2, n)
+    dev = torch.from_numpy(x.view(np.int64)).cuda()
+    root = Z.primitive_root(field, L)
+    stream = torch.cuda.current_stream()
+    sp = stream.cuda_stream
+    b = Z.Batch(ctx, field, [(Z.FORWARD, L, dev.data_ptr() + i * (8 << L), root) for i in range(count)])
+    tuned_ms = b.autotune(reps=5) if not args.no_tune else None
+    ms = timed(lambda: b.execute(stream=sp), args.steps, args.warmup, stream)
+    b.status(stream=sp)
+    # Correctness: one execution of the seeded input, 16 transforms against the oracle.
+    dev.copy_(torch.from_numpy(x.view(np.int64)))
+    b.execute(stream=sp)
+    b.status(stream=sp)
+    got = dev.cpu().numpy().view(np.uint64).reshape(count, 1 << L)
+    ok = True
+    for i in range(0, count, count // 16):
+        row = x[i << L:(i + 1) << L].copy()
+        O.ntt(field, row, L, root)
+        ok = ok and bool(np.array_equal(got[i], row))
+    alg_bytes = 16 * n
+    achieved = alg_bytes / (ms * 1e-3)
+    mad, copy = ctx.probe_peaks()
+    roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK, "traffic": None}
+    cpu = None
+    if not args.skip_cpu:
+        sample = 512
+        cx = x[:sample << L].copy()
+        roots = np.zeros(33, dtype=np.uint64)
+        roots[L] = root[0]
+        offs = np.arange(sample, dtype=np.uint64) << np.uint64(L)
+        t1 = time.time()
+        O.ntt_batch_mt(field, cx, offs, np.full(sample, L, dtype=np.uint64), roots, 1)
+        cs = time.time() - t1
+        cpu = {"value": (sample << L) / cs / 1e6, "unit": "Melem/s", "cores": 1, "kind": "port",
+               "sample": f"{sample} of the 4096 transforms, C oracle, 1 thread, {cs:.1f} s"}
+    line = base_line(f"Melem/s {name} batch 4096 x 2^12 NTT", n / (ms * 1e-3) / 1e6, "Melem/s", args, ms,
+                     {"workload": f"4096 independent {name} forward NTTs of 2^12, one persistent launch",
+                      "field": name.lower(), "count": count, "log_n": L, "elements": n},
+                     f"synthetic: uniform {name} in [0, p), seed 2", "u32")
+    line.update({"roofline": roofline, "cpu_baseline": cpu,
+                 "extras": {"batch_shape": b.shape(), "autotune_ms": tuned_ms, "bit_exact_sample_vs_oracle": ok,
+                            "measured_copy_GB_s": copy / 1e9, "frac_of_measured_copy": achieved / copy}})
+    b.close()
     return line
 
 
@@ -463,7 +520,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="fr-ntt",
-                    choices=["fr-ntt", "gl-batch", "fr-uneven", "serve", "ref-2p20", "poly-mul"])
+                    choices=["fr-ntt", "gl-batch", "bb-batch", "kb-batch", "fr-uneven", "serve", "ref-2p20",
+                             "poly-mul"])
     ap.add_argument("--log-n", type=int, default=24)
     ap.add_argument("--cpu-log-n", type=int, default=24, help="size of the CPU-baseline sample")
     ap.add_argument("--skip-cpu", action="store_true")
@@ -489,6 +547,10 @@ def main() -> int:
         line = run_ref_2p20(args, Z, ctx)
     elif args.workload == "poly-mul":
         line = run_poly_mul(args, Z, ctx)
+    elif args.workload == "bb-batch":
+        line = run_small_batch(args, Z, ctx, Z.FIELD_BB)
+    elif args.workload == "kb-batch":
+        line = run_small_batch(args, Z, ctx, Z.FIELD_KB)
     else:
         line = run_serve(args, Z, ctx)
     print(json.dumps(line), flush=True)

@@ -35,8 +35,9 @@ struct BatchEntry {
 
 // One persistent launch over `nitems` tiles described by `ent[0..nent)`.
 // `uniform_tiles`: the tile count every entry has, when they are all equal
-// (entry i then starts at item i * uniform_tiles), else 0.  `*counter` must
-// be zero when the launch starts.  Tiles hold 2^tile_log elements; the
+// (entry i then starts at item i * uniform_tiles), else 0.  counter[0..2)
+// (next item, finished workgroups) must be zero when the launch starts; the
+// launch's last workgroup leaves them zero again.  Tiles hold 2^tile_log elements; the
 // workgroup has 2^tile_log / 8 threads; the grid is min(grid, nitems)
 // workgroups.
 hipError_t launch_batch_level(int field, const BatchXform *xf, const BatchEntry *ent, uint32_t nent,
@@ -44,18 +45,18 @@ hipError_t launch_batch_level(int field, const BatchXform *xf, const BatchEntry 
                               hipStream_t s);
 // One persistent launch over n transforms that each fit one tile of 2^K
 // elements (K = their log size, 9..11 for BN254 Fr, 9..12 for the others):
-// transform ent[i].xf is item i.  `*counter` must be zero at launch.
+// transform ent[i].xf is item i.  counter[0..2) as for launch_batch_level.
+// `tw_all`: the twiddle table every one of them uses, or null; given it, the
+// small fields keep it in LDS for the whole launch.
 hipError_t launch_whole_fr(int K, const BatchXform *xf, const BatchEntry *ent, uint32_t n, uint32_t *counter,
-                           hipStream_t s);
+                           const uint32_t *tw_all, hipStream_t s);
 hipError_t launch_whole_gl(int K, const BatchXform *xf, const BatchEntry *ent, uint32_t n, uint32_t *counter,
-                           hipStream_t s);
+                           const uint32_t *tw_all, hipStream_t s);
 hipError_t launch_whole_f31(int field, int K, const BatchXform *xf, const BatchEntry *ent, uint32_t n,
-                            uint32_t *counter, hipStream_t s);
+                            uint32_t *counter, const uint32_t *tw_all, hipStream_t s);
 inline int whole_max_log(int field) { return field == 1 ? 11 : 12; }
 constexpr int kWholeMinLog = 9;
 
-// counters[0..n) = 0 by a kernel (capturable in a graph, unlike a memset).
-hipError_t launch_clear_counters(uint32_t *counters, uint32_t n, hipStream_t s);
 hipError_t configure_batch_lds();
 
 // out[i] for i < n: element i of the counter-based stream (seed, stream_id),

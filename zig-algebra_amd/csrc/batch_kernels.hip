@@ -137,7 +137,10 @@ __global__ __launch_bounds__(BLOCK, WAVES_PER_SIMD) void batch_kernel(const Batc
   __syncthreads();
   uint32_t it = s_item;
   for (;;) {
-    if (it >= nitems) return;
+    if (it >= nitems) {
+      if (threadIdx.x == 0) retire_workgroup(next);
+      return;
+    }
     // Claim the following item now, so that the atomic's round trip overlaps
     // this item's work; it is published through s_item at the end.
     uint32_t claimed = 0;
@@ -222,17 +225,6 @@ __global__ __launch_bounds__(BLOCK, WAVES_PER_SIMD) void batch_kernel(const Batc
     __syncthreads();
     it = s_item;
   }
-}
-
-__global__ void clear_counters_kernel(uint32_t *counters, uint32_t n) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) counters[i] = 0;
-}
-
-hipError_t launch_clear_counters(uint32_t *counters, uint32_t n, hipStream_t s) {
-  if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(clear_counters_kernel, dim3((n + 255) / 256), dim3(256), 0, s, counters, n);
-  return hipGetLastError();
 }
 
 hipError_t launch_batch_level(int field, const BatchXform *xf, const BatchEntry *ent, uint32_t nent,

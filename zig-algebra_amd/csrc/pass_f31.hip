@@ -23,9 +23,9 @@ hipError_t launch_transform_f31(int field, int L, uint32_t *data, uint32_t *scra
 }
 
 hipError_t launch_whole_f31(int field, int K, const BatchXform *xf, const BatchEntry *ent, uint32_t n,
-                            uint32_t *counter, hipStream_t s) {
-  if (field == kBb) return launch_whole_t<BbT>(K, xf, ent, n, counter, s);
-  if (field == kKb) return launch_whole_t<KbT>(K, xf, ent, n, counter, s);
+                            uint32_t *counter, const uint32_t *tw_all, hipStream_t s) {
+  if (field == kBb) return launch_whole_t<BbT>(K, xf, ent, n, counter, tw_all, s);
+  if (field == kKb) return launch_whole_t<KbT>(K, xf, ent, n, counter, tw_all, s);
   return hipErrorInvalidValue;
 }
 

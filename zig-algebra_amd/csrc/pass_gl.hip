@@ -15,8 +15,8 @@ hipError_t launch_transform_gl(int L, uint32_t *data, uint32_t *scratch, const u
 }
 
 hipError_t launch_whole_gl(int K, const BatchXform *xf, const BatchEntry *ent, uint32_t n, uint32_t *counter,
-                           hipStream_t s) {
-  return launch_whole_t<GlT>(K, xf, ent, n, counter, s);
+                           const uint32_t *tw_all, hipStream_t s) {
+  return launch_whole_t<GlT>(K, xf, ent, n, counter, tw_all, s);
 }
 
 }  // namespace zntt

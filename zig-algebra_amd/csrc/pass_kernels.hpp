@@ -163,7 +163,9 @@ struct StageMap {
   static constexpr int j(int jj) { return kk(jj) & ((1 << T) - 1); }
 };
 
-template <class F, int R, int l0, bool first_round, bool last_round, bool ODD>
+// TWL: the pass's twiddle table is resident in LDS after the tile's planes
+// (a whole transform: 2^K entries, plane stride 2^K), not read from HBM.
+template <class F, int R, int l0, bool first_round, bool last_round, bool ODD, bool TWL = false>
 __device__ __forceinline__ void pass_round(typename F::E (&x)[8], const PassArgs &a, uint32_t *lds,
                                            const TileCtx &c) {
   using E = typename F::E;
@@ -185,8 +187,11 @@ __device__ __forceinline__ void pass_round(typename F::E (&x)[8], const PassArgs
       const size_t lo = a.first ? 0 : c.lo0 + g[i];
       const int jbase = base[i] & ((1 << l0) - 1);
       sfor<(1 << t)>([&](auto kk) {
-        tw[i * (SET - 1) + (1 << t) - 1 + kk] =
-            F::load_dev(a.tw, off + ((size_t)(jbase | (kk << l0)) << c.lo_bits) + lo);
+        const size_t ti = off + ((size_t)(jbase | (kk << l0)) << c.lo_bits) + lo;
+        if constexpr (TWL)
+          tw[i * (SET - 1) + (1 << t) - 1 + kk] = F::lds_get(lds + F::W * c.tile, c.tile, (int)ti);
+        else
+          tw[i * (SET - 1) + (1 << t) - 1 + kk] = F::load_dev(a.tw, ti);
       });
     });
   });
@@ -301,17 +306,17 @@ __device__ __forceinline__ void pass_round(typename F::E (&x)[8], const PassArgs
 
 // Rounds of a K-stage pass: the first takes K - 3*(ceil(K/3) - 1) stages, the
 // rest three each; unrolled at compile time so `x` stays in registers.
-template <class F, int K, int RI, bool ODD>
+template <class F, int K, int RI, bool ODD, bool TWL = false>
 __device__ __forceinline__ void pass_rounds(typename F::E (&x)[8], const PassArgs &a, uint32_t *lds,
                                             const TileCtx &c) {
   constexpr int NR = (K + 2) / 3;
   constexpr int R0 = K - 3 * (NR - 1);
   constexpr int R = RI == 0 ? R0 : 3;
   constexpr int L0 = RI == 0 ? 0 : R0 + 3 * (RI - 1);
-  pass_round<F, R, L0, RI == 0, RI == NR - 1, ODD>(x, a, lds, c);
+  pass_round<F, R, L0, RI == 0, RI == NR - 1, ODD, TWL>(x, a, lds, c);
   if constexpr (RI + 1 < NR) {
     __syncthreads();
-    pass_rounds<F, K, RI + 1, ODD>(x, a, lds, c);
+    pass_rounds<F, K, RI + 1, ODD, TWL>(x, a, lds, c);
   }
 }
 
@@ -339,17 +344,36 @@ __global__ __launch_bounds__(512) void ntt_pass_r8_kernel(const PassArgs a) {
   pass_rounds<F, K, 0, ODD>(x, a, lds, c);
 }
 
+// Persistent launches claim items from ctr[0] and count finished workgroups
+// in ctr[1].  Thread 0 of each workgroup calls this once it claims no more:
+// its last claim has returned, so when the count reaches the grid size every
+// claim of the launch is done and the last workgroup returns both counters to
+// zero for the next launch on the stream.
+__device__ __forceinline__ void retire_workgroup(uint32_t *ctr) {
+  if (atomicAdd(ctr + 1, 1u) == gridDim.x - 1) {
+    atomicExch(ctr, 0u);
+    atomicExch(ctr + 1, 0u);
+  }
+}
+
 // Persistent kernel over a batch of transforms that each fit one tile: item
 // i is transform ent[i].xf, run as a single pass of K = L stages with the
 // register rounds above (coalesced staging, compile-time indexing).  Items
-// are claimed from `*next`, which must be zero at launch; each workgroup
-// claims its following item before working on the current one.
-template <class F, int K>
+// are claimed from next[0] (next[0..2) zero at launch, and zero again when
+// the launch ends); each workgroup claims its following item before working
+// on the current one.  TWL: every
+// transform of the launch uses the table `tw_all` (2^K entries), which each
+// workgroup copies into LDS once, behind the tile's planes.
+template <class F, int K, bool TWL>
 __global__ __launch_bounds__(512) void whole_batch_kernel(const BatchXform *xf, const BatchEntry *ent, uint32_t n,
-                                                          uint32_t *next) {
+                                                          uint32_t *next, const uint32_t *tw_all) {
   using E = typename F::E;
   extern __shared__ uint32_t lds[];
   __shared__ uint32_t s_item;
+  if constexpr (TWL) {
+    uint32_t *t = lds + F::W * (1 << K);
+    for (int e = threadIdx.x; e < (1 << K); e += blockDim.x) F::lds_put(t, 1 << K, e, F::load_dev(tw_all, e));
+  }
   if (threadIdx.x == 0) s_item = atomicAdd(next, 1u);
   __syncthreads();
   uint32_t it = s_item;
@@ -382,48 +406,60 @@ __global__ __launch_bounds__(512) void whole_batch_kernel(const BatchXform *xf, 
     c.hi_bits = 0;
     c.hi = c.lo0 = c.c0 = 0;
     E x[8];
-    pass_rounds<F, K, 0, true>(x, a, lds, c);
+    pass_rounds<F, K, 0, true, TWL>(x, a, lds, c);
     __syncthreads();  // LDS is reused by the next item
     if (threadIdx.x == 0) s_item = claimed;
     __syncthreads();
     it = s_item;
   }
+  if (threadIdx.x == 0) retire_workgroup(next);
 }
 
-template <class F, int K>
+template <class F, int K, bool TWL>
 static hipError_t launch_whole_k(const BatchXform *xf, const BatchEntry *ent, uint32_t n, uint32_t *counter,
-                                 hipStream_t s) {
+                                 const uint32_t *tw_all, hipStream_t s) {
   static int grid = 0;  // resident workgroups on the whole device
-  const size_t lds = (size_t)F::W * 4 << K;
+  const size_t lds = (size_t)F::W * 4 << K << (TWL ? 1 : 0);
   const unsigned threads = 1u << (K - 3);
   if (grid == 0) {
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&whole_batch_kernel<F, K>),
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&whole_batch_kernel<F, K, TWL>),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
     int per_cu = 0, dev = 0, cus = 0;
     if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
     if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, whole_batch_kernel<F, K>, (int)threads, lds);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, whole_batch_kernel<F, K, TWL>, (int)threads, lds);
     if (e != hipSuccess) return e;
     grid = std::max(1, per_cu) * std::max(1, cus);
   }
   const unsigned g = (unsigned)std::min<uint32_t>(n, (uint32_t)grid);
-  hipLaunchKernelGGL((whole_batch_kernel<F, K>), dim3(g), dim3(threads), lds, s, xf, ent, n, counter);
+  hipLaunchKernelGGL((whole_batch_kernel<F, K, TWL>), dim3(g), dim3(threads), lds, s, xf, ent, n, counter, tw_all);
   return hipGetLastError();
+}
+
+template <class F, int K>
+static hipError_t launch_whole_kt(const BatchXform *xf, const BatchEntry *ent, uint32_t n, uint32_t *counter,
+                                  const uint32_t *tw_all, hipStream_t s) {
+  // The LDS-resident table doubles the tile's LDS; BN254 Fr's 72 KiB tile
+  // would then fit once per CU, so it reads its twiddles from HBM.
+  if constexpr (F::W <= 2) {
+    if (tw_all) return launch_whole_k<F, K, true>(xf, ent, n, counter, tw_all, s);
+  }
+  return launch_whole_k<F, K, false>(xf, ent, n, counter, nullptr, s);
 }
 
 // Transforms of 2^9 up to the field's largest register-round tile (BN254 Fr
 // 2^11, the other fields 2^12).
 template <class F>
 static hipError_t launch_whole_t(int K, const BatchXform *xf, const BatchEntry *ent, uint32_t n, uint32_t *counter,
-                                 hipStream_t s) {
+                                 const uint32_t *tw_all, hipStream_t s) {
   if (n == 0) return hipSuccess;
   switch (K) {
-    case 9: return launch_whole_k<F, 9>(xf, ent, n, counter, s);
-    case 10: return launch_whole_k<F, 10>(xf, ent, n, counter, s);
-    case 11: return launch_whole_k<F, 11>(xf, ent, n, counter, s);
+    case 9: return launch_whole_kt<F, 9>(xf, ent, n, counter, tw_all, s);
+    case 10: return launch_whole_kt<F, 10>(xf, ent, n, counter, tw_all, s);
+    case 11: return launch_whole_kt<F, 11>(xf, ent, n, counter, tw_all, s);
     case 12:
-      if constexpr (F::W <= 2) return launch_whole_k<F, 12>(xf, ent, n, counter, s);
+      if constexpr (F::W <= 2) return launch_whole_kt<F, 12>(xf, ent, n, counter, tw_all, s);
       return hipErrorInvalidValue;
     default: return hipErrorInvalidValue;
   }

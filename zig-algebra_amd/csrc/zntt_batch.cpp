@@ -51,6 +51,7 @@ struct zntt_batch {
   struct Launch {
     int whole_log;
     uint32_t ent_off, nent, nitems, uniform;
+    const uint32_t *tw_all;  // whole launches: the one table all transforms use, else null
   };
   std::vector<Launch> launches;
   zntt::BatchXform *d_xf = nullptr;
@@ -123,20 +124,26 @@ int build_plan(zntt_batch *b, const Shape &shape) {
       if (uniform == 0) uniform = x.tiles[p];
       equal = equal && uniform == x.tiles[p];
     }
-    if (first) b->launches.push_back({0, off, (uint32_t)ent.size() - off, first, equal ? uniform : 0});
+    if (first) b->launches.push_back({0, off, (uint32_t)ent.size() - off, first, equal ? uniform : 0, nullptr});
     if (p == 0) {
       for (int L = zntt::whole_max_log(b->field); L >= zntt::kWholeMinLog; L--) {
         const uint32_t woff = (uint32_t)ent.size();
         uint32_t k = 0;
+        const uint32_t *tw = nullptr;
+        bool same = true;
         for (uint32_t t : idx)
-          if (whole(b->xf[t]) && b->xf[t].L == L) ent.push_back(zntt::BatchEntry{t, 0, k++, 0});
-        if (k) b->launches.push_back({L, woff, k, k, 1});
+          if (whole(b->xf[t]) && b->xf[t].L == L) {
+            ent.push_back(zntt::BatchEntry{t, 0, k++, 0});
+            if (!tw) tw = b->xf[t].tw;
+            same = same && tw == b->xf[t].tw;
+          }
+        if (k) b->launches.push_back({L, woff, k, k, 1, same ? tw : nullptr});
       }
     }
   }
   const size_t xf_bytes = b->xf.size() * sizeof(zntt::BatchXform);
   const size_t it_bytes = ent.size() * sizeof(zntt::BatchEntry);
-  const size_t st_bytes = b->launches.size() * sizeof(uint32_t);
+  const size_t st_bytes = 2 * b->launches.size() * sizeof(uint32_t);  // next item, finished workgroups
   auto up = [](size_t v) { return (v + 255) & ~(size_t)255; };
   b->dev_bytes = up(xf_bytes) + up(it_bytes) + up(st_bytes);
   int st = dev_alloc(c, b->dev_bytes, &b->dev);
@@ -146,6 +153,9 @@ int build_plan(zntt_batch *b, const Shape &shape) {
   b->d_ent = (zntt::BatchEntry *)(base + up(xf_bytes));
   b->d_counters = (uint32_t *)(base + up(xf_bytes) + up(it_bytes));
   hipError_t e = hipMemcpyAsync(b->d_xf, b->xf.data(), xf_bytes, hipMemcpyHostToDevice, c->stream);
+  // Counters start at zero; every launch returns its own to zero, so replays
+  // need no clearing node in the graph.
+  if (e == hipSuccess) e = hipMemsetAsync(b->d_counters, 0, st_bytes, c->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(b->d_ent, ent.data(), it_bytes, hipMemcpyHostToDevice, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   if (e != hipSuccess) return hip_status(e);
@@ -157,21 +167,21 @@ int build_plan(zntt_batch *b, const Shape &shape) {
   return ZNTT_OK;
 }
 
-// Clears the launch counters, then the persistent launches in order.
+// The persistent launches in order (each leaves its counters zero).
 hipError_t enqueue(zntt_batch *b, hipStream_t s) {
-  hipError_t e = zntt::launch_clear_counters(b->d_counters, (uint32_t)b->launches.size(), s);
+  hipError_t e = hipSuccess;
   for (size_t i = 0; i < b->launches.size() && e == hipSuccess; i++) {
     const zntt_batch::Launch &l = b->launches[i];
     if (l.whole_log == 0)
       e = zntt::launch_batch_level(b->field, b->d_xf, b->d_ent + l.ent_off, l.nent, l.nitems, l.uniform,
-                                   b->d_counters + i, b->grid, b->shape.tile_log, s);
+                                   b->d_counters + 2 * i, b->grid, b->shape.tile_log, s);
     else if (b->field == ZNTT_FIELD_BN254_FR)
-      e = zntt::launch_whole_fr(l.whole_log, b->d_xf, b->d_ent + l.ent_off, l.nent, b->d_counters + i, s);
+      e = zntt::launch_whole_fr(l.whole_log, b->d_xf, b->d_ent + l.ent_off, l.nent, b->d_counters + 2 * i, l.tw_all, s);
     else if (b->field == ZNTT_FIELD_GOLDILOCKS)
-      e = zntt::launch_whole_gl(l.whole_log, b->d_xf, b->d_ent + l.ent_off, l.nent, b->d_counters + i, s);
+      e = zntt::launch_whole_gl(l.whole_log, b->d_xf, b->d_ent + l.ent_off, l.nent, b->d_counters + 2 * i, l.tw_all, s);
     else
-      e = zntt::launch_whole_f31(b->field, l.whole_log, b->d_xf, b->d_ent + l.ent_off, l.nent, b->d_counters + i,
-                                 s);
+      e = zntt::launch_whole_f31(b->field, l.whole_log, b->d_xf, b->d_ent + l.ent_off, l.nent, b->d_counters + 2 * i,
+                                 l.tw_all, s);
   }
   return e;
 }

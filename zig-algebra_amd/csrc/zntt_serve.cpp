@@ -526,7 +526,7 @@ struct HipDevice : Device {
     if (!sl) return -ZNTT_DEVICE_ERROR;  // more steps in flight than slots: a scheduler bug
     const size_t n = items.size();
     uint8_t *h = (uint8_t *)sl->host;
-    memset(h, 0, 256);  // the step's item counter starts at zero
+    memset(h, 0, 256);  // the step's counters (next item, finished workgroups) start at zero
     auto *hx = (zntt::BatchXform *)(h + 256);
     auto *he = (zntt::BatchEntry *)(h + 256 + align_up(sizeof(zntt::BatchXform) * n, 256));
     uint32_t first = 0, uniform = 0;
