@@ -24,6 +24,8 @@
  *                         libs/curve/src/bn254.zig:67.
  *   ZNTT_FIELD_GOLDILOCKS  8 bytes: the canonical u64 value
  *                         (libs/field/src/field.zig:142), p = 2^64 - 2^32 + 1.
+ *   ZNTT_FIELD_BABYBEAR,   8 bytes: the canonical u64 value, as Goldilocks;
+ *   ZNTT_FIELD_KOALABEAR   p = 2^31 - 2^27 + 1 and 2^31 - 2^24 + 1.
  *
  * Semantics:
  *   - Host-buffer calls are synchronous and in place, like the Zig functions:

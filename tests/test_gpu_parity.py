@@ -354,3 +354,23 @@ def test_device_resident_execute(Z, ctx):
         else:
             fn(F, exp, log_n, root)
         assert np.array_equal(d.cpu().numpy().view(np.uint64).reshape(exp.shape), exp), direction
+
+
+def test_context_budget_exhaustion(Z):
+    """A context whose HBM budget cannot hold a transform's tables and
+    staging reports OutOfMemory (the reference's error.OutOfMemory,
+    root.zig:112) and stays usable for transforms that fit."""
+    small = Z.Context(0, hbm_budget_bytes=1 << 20)
+    try:
+        F = O.FIELD_FR
+        x = O.gen_uniform(F, 5, 1 << 16)  # 2 MiB of data alone
+        with pytest.raises(Z.OutOfMemory):
+            small.ntt(F, x.copy(), 16, O.primitive_root(F, 16)[1])
+        y = O.gen_uniform(F, 6, 1 << 8)
+        exp = y.copy()
+        O.ntt(F, exp, 8, O.primitive_root(F, 8)[1])
+        small.ntt(F, y, 8, O.primitive_root(F, 8)[1])
+        assert np.array_equal(y, exp)
+        assert small.bytes_in_use() <= 1 << 20
+    finally:
+        small.close()

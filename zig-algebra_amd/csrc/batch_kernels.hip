@@ -12,6 +12,7 @@
 // release/acquire between workgroups compiles to an L2 write-back and an L2
 // invalidate per dependency -- measured at 10x the batch's compute time.)
 #include <algorithm>
+#include <mutex>
 
 #include "batch.hpp"
 #include "pass_kernels.hpp"
@@ -258,21 +259,23 @@ hipError_t launch_batch_level(int field, const BatchXform *xf, const BatchEntry 
 }
 
 hipError_t configure_batch_lds() {
-  static bool done = false;
-  if (done) return hipSuccess;
-  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&batch_kernel<FrT, 256, 2>),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, 9 * 4 * (1 << 11));
-  if (e == hipSuccess)
-    e = hipFuncSetAttribute(reinterpret_cast<const void *>(&batch_kernel<GlT, 512, 4>),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, 2 * 4 * (1 << 12));
-  if (e == hipSuccess)
-    e = hipFuncSetAttribute(reinterpret_cast<const void *>(&batch_kernel<BbT, 512, 4>),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, 4 * (1 << 12));
-  if (e == hipSuccess)
-    e = hipFuncSetAttribute(reinterpret_cast<const void *>(&batch_kernel<KbT, 512, 4>),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, 4 * (1 << 12));
-  if (e == hipSuccess) done = true;
-  return e;
+  static std::once_flag once;
+  static hipError_t result = hipSuccess;
+  std::call_once(once, [] {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&batch_kernel<FrT, 256, 2>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, 9 * 4 * (1 << 11));
+    if (e == hipSuccess)
+      e = hipFuncSetAttribute(reinterpret_cast<const void *>(&batch_kernel<GlT, 512, 4>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, 2 * 4 * (1 << 12));
+    if (e == hipSuccess)
+      e = hipFuncSetAttribute(reinterpret_cast<const void *>(&batch_kernel<BbT, 512, 4>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, 4 * (1 << 12));
+    if (e == hipSuccess)
+      e = hipFuncSetAttribute(reinterpret_cast<const void *>(&batch_kernel<KbT, 512, 4>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, 4 * (1 << 12));
+    result = e;
+  });
+  return result;
 }
 
 // ---------------------------------------------------------------------------

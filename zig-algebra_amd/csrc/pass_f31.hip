@@ -1,4 +1,6 @@
 // BabyBear and KoalaBear instantiations of the pass kernels.
+#include <mutex>
+
 #include "pass_kernels.hpp"
 
 namespace zntt {
@@ -6,12 +8,10 @@ namespace zntt {
 template <class F>
 static hipError_t launch_f31_t(int field, int L, uint32_t *data, uint32_t *scratch, const uint32_t *tw,
                                const uint32_t *coset, const uint32_t *scale_dev, bool tw_one, hipStream_t stream) {
-  static bool configured = false;
-  if (!configured) {
-    hipError_t e = configure_pass_lds<F>(4 * (1 << 12));
-    if (e != hipSuccess) return e;
-    configured = true;
-  }
+  static std::once_flag once;
+  static hipError_t configured = hipSuccess;
+  std::call_once(once, [] { configured = configure_pass_lds<F>(4 * (1 << 12)); });
+  if (configured != hipSuccess) return configured;
   return run_passes_t<F>(field, L, data, scratch, tw, coset, scale_dev, tw_one, stream);
 }
 

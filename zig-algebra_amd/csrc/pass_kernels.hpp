@@ -8,6 +8,7 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <mutex>
 #include <utility>
 
 #include "batch.hpp"
@@ -418,20 +419,24 @@ __global__ __launch_bounds__(512) void whole_batch_kernel(const BatchXform *xf, 
 template <class F, int K, bool TWL>
 static hipError_t launch_whole_k(const BatchXform *xf, const BatchEntry *ent, uint32_t n, uint32_t *counter,
                                  const uint32_t *tw_all, hipStream_t s) {
-  static int grid = 0;  // resident workgroups on the whole device
   const size_t lds = (size_t)F::W * 4 << K << (TWL ? 1 : 0);
   const unsigned threads = 1u << (K - 3);
-  if (grid == 0) {
+  // Resident workgroups on the whole device, computed once.
+  static std::once_flag once;
+  static int grid = 0;
+  static hipError_t setup = hipSuccess;
+  std::call_once(once, [&] {
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&whole_batch_kernel<F, K, TWL>),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (e != hipSuccess) return e;
     int per_cu = 0, dev = 0, cus = 0;
-    if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
-    if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, whole_batch_kernel<F, K, TWL>, (int)threads, lds);
-    if (e != hipSuccess) return e;
+    if (e == hipSuccess) e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e == hipSuccess)
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, whole_batch_kernel<F, K, TWL>, (int)threads, lds);
     grid = std::max(1, per_cu) * std::max(1, cus);
-  }
+    setup = e;
+  });
+  if (setup != hipSuccess) return setup;
   const unsigned g = (unsigned)std::min<uint32_t>(n, (uint32_t)grid);
   hipLaunchKernelGGL((whole_batch_kernel<F, K, TWL>), dim3(g), dim3(threads), lds, s, xf, ent, n, counter, tw_all);
   return hipGetLastError();
