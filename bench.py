@@ -434,6 +434,54 @@ def run_poly_mul(args, Z, ctx):
     return line
 
 
+BLAKE3_OPS = 7 * 8 * 12  # VALU ops of one BLAKE3 compression: 7 rounds x 8 quarter-rounds x 12
+
+
+def run_merkle(args, Z, ctx):
+    """Merkle commitment of one FRI layer (libs/fri/src/root.zig:378-384 with
+    MerkleTree.initFromHashes, merkle_tree.zig:147-182): 2^log_n BN254 Fr
+    evaluations resident in HBM, BLAKE3 leaves over the antipodal pairs, the
+    whole tree written back to HBM (the prover keeps it for query paths)."""
+    import numpy as np
+    import torch
+    F, L = Z.FIELD_FR, args.log_n
+    n = 1 << L
+    ev = Z.fill_uniform(F, 21, n)
+    stream = torch.cuda.current_stream()
+    sp = stream.cuda_stream
+    tev = torch.from_numpy(ev.view(np.int64)).cuda()
+    nodes = torch.empty(n * 4, dtype=torch.int64, device="cuda")
+    roots = []
+    ms = timed(lambda: roots.append(ctx.merkle_commit_device(F, tev.data_ptr(), L, nodes.data_ptr(), stream=sp)),
+               args.steps, args.warmup, stream)
+    root_only = ctx.merkle_commit_device(F, tev.data_ptr(), L, stream=sp)
+    compressions = n - 1  # n/2 leaves + n/2 - 1 inner nodes
+    achieved = compressions * BLAKE3_OPS / (ms * 1e-3)
+    roofline = {"bound": "valu", "achieved": achieved / 1e12, "peak": VALU_MUL_PEAK / 1e12, "unit": "Tvalu32/s",
+                "frac": achieved / VALU_MUL_PEAK, "traffic": None}
+    cpu, oracle_match = None, None
+    if not args.skip_cpu:
+        import oracle as O
+        t1 = time.time()
+        _, want = O.merkle_commit(O.FIELD_FR, ev)
+        cs = time.time() - t1
+        oracle_match = want == roots[-1]
+        cpu = {"value": n / cs / 1e6, "unit": "Melem/s", "cores": 1, "kind": "port",
+               "sample": f"the same 2^{L} evaluations committed by the C oracle (BLAKE3 from its "
+                         f"specification), 1 thread, {cs:.1f} s"}
+    line = base_line(f"Melem/s BN254 Fr FRI layer commitment 2^{L} (BLAKE3 Merkle tree)", n / (ms * 1e-3) / 1e6,
+                     "Melem/s", args, ms,
+                     {"workload": f"Merkle commitment of 2^{L} BN254 Fr evaluations resident in HBM: 2^{L - 1} "
+                                  "BLAKE3 leaves over antipodal pairs, whole tree written to HBM, root to host",
+                      "field": "bn254_fr", "evaluations": n},
+                     "synthetic: uniform BN254 Fr evaluations, seed 21", "u32")
+    line.update({"roofline": roofline, "cpu_baseline": cpu,
+                 "extras": {"hbm_gb_s": 2 * n * 32 / (ms * 1e-3) / 1e9, "compressions": compressions,
+                            "root_only_equal": root_only == roots[-1],
+                            "steps_agree": len(set(roots)) == 1, "root_equals_oracle": oracle_match}})
+    return line
+
+
 def serve_muls(arrivals):
     """Algorithmic 32-bit multiplies of a schedule: per request (N/2) log2 N
     butterfly products plus 2^k coset products, N = 2^(k+b)."""
@@ -521,7 +569,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="fr-ntt",
                     choices=["fr-ntt", "gl-batch", "bb-batch", "kb-batch", "fr-uneven", "serve", "ref-2p20",
-                             "poly-mul"])
+                             "poly-mul", "merkle"])
     ap.add_argument("--log-n", type=int, default=24)
     ap.add_argument("--cpu-log-n", type=int, default=24, help="size of the CPU-baseline sample")
     ap.add_argument("--skip-cpu", action="store_true")
@@ -551,6 +599,8 @@ def main() -> int:
         line = run_small_batch(args, Z, ctx, Z.FIELD_BB)
     elif args.workload == "kb-batch":
         line = run_small_batch(args, Z, ctx, Z.FIELD_KB)
+    elif args.workload == "merkle":
+        line = run_merkle(args, Z, ctx)
     else:
         line = run_serve(args, Z, ctx)
     print(json.dumps(line), flush=True)

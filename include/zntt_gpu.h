@@ -356,6 +356,37 @@ int zntt_poly_mul(zntt_ctx *ctx, int field, const void *a, size_t a_len, const v
 int zntt_poly_mul_device(zntt_ctx *ctx, int field, const void *a, size_t a_len, const void *b, size_t b_len,
                          void *out, void *stream);
 
+/* ------------------------------------------------------------------------ */
+/* Merkle commitment of an LDE / FRI layer (BLAKE3).                         */
+/*                                                                           */
+/* The FRI prover commits a layer f of n = 2^log_n evaluations (natural      */
+/* domain order) by a Merkle tree over n/2 leaves                            */
+/*   leaf j = BLAKE3(toBytes(f[j]) || toBytes(f[j + n/2]))                   */
+/* (libs/fri/src/root.zig:283-292, :378-384), built as                       */
+/* MerkleTree.initFromHashes (libs/merkle/src/merkle_tree.zig:147-182).      */
+/* toBytes is the canonical value, little-endian: 32 bytes for BN254 Fr      */
+/* (out of the Montgomery form), 8 for Goldilocks, 4 for BabyBear and        */
+/* KoalaBear.  The tree is the reference's node array: 2 * cap nodes of 32   */
+/* bytes, node 0 zero, node 1 the root, leaf i at node cap + i, BLAKE3("")   */
+/* leaves past the last real one.                                            */
+/* Device buffers here are 16-byte aligned (ZNTT_INVALID_ARGUMENT if not).   */
+/* ------------------------------------------------------------------------ */
+
+/* Commits one layer: `evals` is a device buffer of 2^log_n elements in the
+ * storage layout, 1 <= log_n <= ZNTT_MAX_LOG_N.  `nodes` is null or a device
+ * buffer of 2^log_n * 32 bytes that receives the whole tree (cap = n/2).  The
+ * root's 32 bytes are written to host memory `root` once the work enqueued
+ * on `stream` has finished (the call waits for it).  Replaces the commit of
+ * fri/src/root.zig:378-384 (leaves + initFromHashes + tree.root()). */
+int zntt_merkle_commit(zntt_ctx *ctx, int field, const void *evals, size_t log_n, void *nodes, void *root,
+                       void *stream);
+
+/* MerkleTree.initFromHashes over `count` >= 1 leaf hashes (device buffer of
+ * count * 32 bytes); `nodes` null or a device buffer of 2 * cap * 32 bytes,
+ * cap = count rounded up to a power of two (at most 2^ZNTT_MAX_LOG_N).  The
+ * root goes to host memory `root` as above (merkle_tree.zig:147-182, :189). */
+int zntt_merkle_from_hashes(zntt_ctx *ctx, const void *hashes, size_t count, void *nodes, void *root, void *stream);
+
 /* Blocks until every call enqueued on the context's own stream is done. */
 int zntt_ctx_synchronize(zntt_ctx *ctx);
 

@@ -81,6 +81,11 @@ def lib():
         L.oracle_xoshiro_stream.argtypes = [u64, vp, sz]
         L.oracle_xoshiro_stream.restype = None
         L.oracle_ntt_batch_mt.argtypes = [i32, vp, vp, vp, vp, sz, i32]
+        L.oracle_blake3.argtypes = [ctypes.c_char_p, sz, vp]
+        L.oracle_blake3.restype = None
+        L.oracle_fri_leaves.argtypes = [i32, vp, sz, vp]
+        L.oracle_merkle_from_hashes.argtypes = [vp, sz, vp, vp]
+        L.oracle_merkle_commit.argtypes = [i32, vp, sz, vp, vp]
         L.oracle_init()
         _lib = L
     return _lib
@@ -259,3 +264,47 @@ def ntt_batch_mt(field: int, data: np.ndarray, offsets: np.ndarray, log_ns: np.n
     log_ns = np.ascontiguousarray(log_ns, dtype=np.uint64)
     return lib().oracle_ntt_batch_mt(field, _ptr(data), _ptr(offsets), _ptr(log_ns),
                                      _ptr(roots_by_log), len(log_ns), threads)
+
+
+def _bptr(a: np.ndarray):
+    assert isinstance(a, np.ndarray) and a.flags["C_CONTIGUOUS"] and a.dtype == np.uint8
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def blake3(data: bytes) -> bytes:
+    """BLAKE3 of `data`, 32-byte output."""
+    out = np.zeros(32, dtype=np.uint8)
+    lib().oracle_blake3(bytes(data), len(data), _bptr(out))
+    return out.tobytes()
+
+
+def fri_leaves(field: int, evals: np.ndarray) -> np.ndarray:
+    """Leaf hashes of one FRI layer: (n/2, 32) uint8."""
+    n = evals.shape[0]
+    out = np.zeros((max(n // 2, 1), 32), dtype=np.uint8)
+    st = lib().oracle_fri_leaves(field, _ptr(np.ascontiguousarray(evals)), n, _bptr(out))
+    assert st == OK, st
+    return out
+
+
+def merkle_from_hashes(hashes: np.ndarray) -> tuple[np.ndarray, bytes]:
+    """MerkleTree.initFromHashes: (nodes (2*cap, 32) uint8, root)."""
+    hashes = np.ascontiguousarray(hashes, dtype=np.uint8).reshape(-1, 32)
+    cap = 1
+    while cap < hashes.shape[0]:
+        cap *= 2
+    nodes = np.zeros((2 * cap, 32), dtype=np.uint8)
+    root = np.zeros(32, dtype=np.uint8)
+    st = lib().oracle_merkle_from_hashes(_bptr(hashes), hashes.shape[0], _bptr(nodes), _bptr(root))
+    assert st == OK, st
+    return nodes, root.tobytes()
+
+
+def merkle_commit(field: int, evals: np.ndarray) -> tuple[np.ndarray, bytes]:
+    """Commitment of one FRI layer of n = 2^k values: (nodes (n, 32) uint8, root)."""
+    n = evals.shape[0]
+    nodes = np.zeros((n, 32), dtype=np.uint8)
+    root = np.zeros(32, dtype=np.uint8)
+    st = lib().oracle_merkle_commit(field, _ptr(np.ascontiguousarray(evals)), n, _bptr(nodes), _bptr(root))
+    assert st == OK, st
+    return nodes, root.tobytes()

@@ -998,3 +998,183 @@ int oracle_ntt_batch_mt(int field, void *data, const size_t *offsets, const size
   pthread_mutex_destroy(&j.mu);
   return j.status;
 }
+
+/* ------------------------------------------------------------------------ */
+/* BLAKE3 and the FRI layer commitment.                                      */
+/*                                                                           */
+/* The FRI prover commits each layer f (n = 2^k values, natural order) by a  */
+/* Merkle tree over n/2 leaves, leaf j = BLAKE3(toBytes(f[j]) ||             */
+/* toBytes(f[j + n/2])) (libs/fri/src/root.zig:283-292, :378-384), built as  */
+/* MerkleTree.initFromHashes (libs/merkle/src/merkle_tree.zig:147-182): the  */
+/* leaf count is rounded up to a power of two with BLAKE3("") leaves,        */
+/* nodes[cap + i] = leaf i, nodes[s + i] = BLAKE3(nodes[2(s+i)] ||           */
+/* nodes[2(s+i)+1]) for each level start s, root = nodes[1], nodes[0] = 0.   */
+/* toBytes is the canonical value, little-endian, in (bits + 7) / 8 bytes    */
+/* (libs/field/src/field.zig:244-252 small fields, :1002-1016 Fr).           */
+/*                                                                           */
+/* The hash is std.crypto.hash.Blake3 of the Zig standard library (the FRI   */
+/* and Merkle code import it, fri/src/root.zig:73), i.e. BLAKE3 as its       */
+/* specification defines it: 64-byte blocks compressed by 7 rounds of the    */
+/* quarter-round G over a 16-word state, 1024-byte chunks, chunk chaining    */
+/* values merged pairwise into a binary tree, the last compression flagged   */
+/* ROOT.  Pinned by the reference's BLAKE3 known-answer vectors              */
+/* (libs/hash/src/root.zig:121-127, :173-247).                               */
+/* ------------------------------------------------------------------------ */
+
+enum { B3_CHUNK_START = 1, B3_CHUNK_END = 2, B3_PARENT = 4, B3_ROOT = 8 };
+static const uint32_t B3_IV[8] = {0x6A09E667u, 0xBB67AE85u, 0x3C6EF372u, 0xA54FF53Au,
+                                  0x510E527Fu, 0x9B05688Cu, 0x1F83D9ABu, 0x5BE0CD19u};
+/* Word i of the next round's message is word B3_PERM[i] of this round's. */
+static const uint8_t B3_PERM[16] = {2, 6, 3, 10, 7, 0, 4, 13, 1, 11, 12, 5, 9, 14, 15, 8};
+/* The four column and four diagonal quarter-rounds of one round. */
+static const uint8_t B3_QR[8][4] = {{0, 4, 8, 12}, {1, 5, 9, 13}, {2, 6, 10, 14}, {3, 7, 11, 15},
+                                    {0, 5, 10, 15}, {1, 6, 11, 12}, {2, 7, 8, 13}, {3, 4, 9, 14}};
+
+static inline uint32_t ror32(uint32_t x, int n) { return (x >> n) | (x << (32 - n)); }
+
+static void b3_compress(const uint32_t cv[8], const uint32_t block[16], uint64_t counter, uint32_t len,
+                        uint32_t flags, uint32_t out[16]) {
+  uint32_t v[16], m[16], t[16];
+  for (int i = 0; i < 8; i++) { v[i] = cv[i]; v[8 + i] = i < 4 ? B3_IV[i] : 0; }
+  v[12] = (uint32_t)counter;
+  v[13] = (uint32_t)(counter >> 32);
+  v[14] = len;
+  v[15] = flags;
+  memcpy(m, block, sizeof m);
+  for (int r = 0; r < 7; r++) {
+    for (int q = 0; q < 8; q++) {
+      const int a = B3_QR[q][0], b = B3_QR[q][1], c = B3_QR[q][2], d = B3_QR[q][3];
+      v[a] += v[b] + m[2 * q];
+      v[d] = ror32(v[d] ^ v[a], 16);
+      v[c] += v[d];
+      v[b] = ror32(v[b] ^ v[c], 12);
+      v[a] += v[b] + m[2 * q + 1];
+      v[d] = ror32(v[d] ^ v[a], 8);
+      v[c] += v[d];
+      v[b] = ror32(v[b] ^ v[c], 7);
+    }
+    for (int i = 0; i < 16; i++) t[i] = m[B3_PERM[i]];
+    memcpy(m, t, sizeof m);
+  }
+  for (int i = 0; i < 8; i++) {
+    out[i] = v[i] ^ v[8 + i];
+    out[8 + i] = v[8 + i] ^ cv[i];
+  }
+}
+
+static void b3_words(const uint8_t *p, size_t len, uint32_t w[16]) {
+  uint8_t blk[64] = {0};
+  memcpy(blk, p, len);
+  for (int i = 0; i < 16; i++)
+    w[i] = (uint32_t)blk[4 * i] | (uint32_t)blk[4 * i + 1] << 8 | (uint32_t)blk[4 * i + 2] << 16 |
+           (uint32_t)blk[4 * i + 3] << 24;
+}
+
+static void b3_parent(const uint32_t l[8], const uint32_t r[8], uint32_t flags, uint32_t cv[8]) {
+  uint32_t blk[16], o[16];
+  memcpy(blk, l, 32);
+  memcpy(blk + 8, r, 32);
+  b3_compress(B3_IV, blk, 0, 64, B3_PARENT | flags, o);
+  memcpy(cv, o, 32);
+}
+
+/* One-shot BLAKE3 with a 32-byte output (unkeyed). */
+void oracle_blake3(const uint8_t *in, size_t len, uint8_t out[32]) {
+  uint32_t stack[64][8];
+  int depth = 0;
+  uint64_t chunk = 0;
+  size_t pos = 0;
+  for (;;) {
+    const size_t rest = len - pos;
+    const int last = rest <= 1024;
+    const size_t clen = last ? rest : 1024;
+    const size_t nblk = clen == 0 ? 1 : (clen + 63) / 64;
+    uint32_t cv[8];
+    memcpy(cv, B3_IV, sizeof cv);
+    for (size_t b = 0; b < nblk; b++) {
+      const size_t blen = clen - b * 64 < 64 ? clen - b * 64 : 64;
+      uint32_t w[16], o[16];
+      b3_words(in + pos + b * 64, blen, w);
+      uint32_t flags = (b == 0 ? B3_CHUNK_START : 0) | (b + 1 == nblk ? B3_CHUNK_END : 0);
+      if (last && depth == 0 && b + 1 == nblk) flags |= B3_ROOT;
+      b3_compress(cv, w, chunk, (uint32_t)blen, flags, o);
+      memcpy(cv, o, sizeof cv);
+    }
+    if (last) {
+      /* The final chunk closes every subtree still open on the stack. */
+      while (depth > 0) {
+        depth--;
+        b3_parent(stack[depth], cv, depth == 0 ? B3_ROOT : 0, cv);
+      }
+      for (int i = 0; i < 8; i++)
+        for (int k = 0; k < 4; k++) out[4 * i + k] = (uint8_t)(cv[i] >> (8 * k));
+      return;
+    }
+    /* A chunk followed by more input: merge every completed subtree. */
+    chunk++;
+    for (uint64_t total = chunk; (total & 1) == 0; total >>= 1) b3_parent(stack[--depth], cv, 0, cv);
+    memcpy(stack[depth++], cv, sizeof cv);
+    pos += 1024;
+  }
+}
+
+/* toBytes of element i: the canonical value, little-endian; returns the byte count. */
+static int to_bytes(int field, const void *elems, size_t i, uint8_t *out) {
+  if (field == FIELD_FR) {
+    fr_t x;
+    memcpy(&x, (const uint8_t *)elems + 32 * i, 32);
+    x = fr_from_mont(x);
+    for (int k = 0; k < 4; k++)
+      for (int b = 0; b < 8; b++) out[8 * k + b] = (uint8_t)(x.l[k] >> (8 * b));
+    return 32;
+  }
+  sf_ctx c;
+  if (!sf_for(field, &c)) return 0;
+  const uint64_t v = ((const uint64_t *)elems)[i];
+  const int nb = (c.bits + 7) / 8;
+  for (int b = 0; b < nb; b++) out[b] = (uint8_t)(v >> (8 * b));
+  return nb;
+}
+
+/* leaves[j] = BLAKE3(toBytes(f[j]) || toBytes(f[j + n/2])), j < n/2. */
+int oracle_fri_leaves(int field, const void *evals, size_t n, uint8_t *leaves) {
+  fr_init();
+  if (oracle_elem_size(field) == 0) return OR_INVALID_ARGUMENT;
+  if (n < 2 || (n & 1)) return OR_INVALID_LENGTH;
+  const size_t half = n / 2;
+  for (size_t j = 0; j < half; j++) {
+    uint8_t buf[64];
+    const int a = to_bytes(field, evals, j, buf);
+    const int b = to_bytes(field, evals, j + half, buf + a);
+    oracle_blake3(buf, (size_t)(a + b), leaves + 32 * j);
+  }
+  return OR_OK;
+}
+
+/* MerkleTree.initFromHashes: nodes holds 2 * cap * 32 bytes, cap the leaf
+ * count rounded up to a power of two. */
+int oracle_merkle_from_hashes(const uint8_t *hashes, size_t count, uint8_t *nodes, uint8_t root[32]) {
+  if (count == 0) return OR_INVALID_LENGTH;
+  size_t cap = 1;
+  while (cap < count) cap <<= 1;
+  memset(nodes, 0, 32);
+  memcpy(nodes + 32 * cap, hashes, 32 * count);
+  uint8_t empty[32];
+  oracle_blake3((const uint8_t *)"", 0, empty);
+  for (size_t i = count; i < cap; i++) memcpy(nodes + 32 * (cap + i), empty, 32);
+  for (size_t s = cap / 2; s >= 1; s /= 2)
+    for (size_t i = 0; i < s; i++) oracle_blake3(nodes + 64 * (s + i), 64, nodes + 32 * (s + i));
+  memcpy(root, nodes + 32, 32);
+  return OR_OK;
+}
+
+/* The commitment of one FRI layer of n = 2^k values: nodes holds n * 32 bytes. */
+int oracle_merkle_commit(int field, const void *evals, size_t n, uint8_t *nodes, uint8_t root[32]) {
+  if (n < 2 || (n & (n - 1))) return OR_INVALID_LENGTH;
+  uint8_t *leaves = (uint8_t *)malloc(16 * n);
+  if (!leaves) return OR_OUT_OF_MEMORY;
+  int st = oracle_fri_leaves(field, evals, n, leaves);
+  if (st == OR_OK) st = oracle_merkle_from_hashes(leaves, n / 2, nodes, root);
+  free(leaves);
+  return st;
+}

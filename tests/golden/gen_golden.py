@@ -11,7 +11,10 @@ Two kinds of fixtures are written:
      * ``ref_ntt_kat.json``  -- the DFT known-answer cases of
        libs/ntt/src/root.zig:438-449 (Goldilocks and BabyBear, log_n 1..4);
      * ``ref_field_vectors.json`` -- the add/mul/inv/neg vectors of
-       libs/field/src/lib.zig:257-276 (BabyBear and Goldilocks rows).
+       libs/field/src/lib.zig:257-276 (BabyBear and Goldilocks rows);
+     * ``ref_blake3_kat.json`` -- the BLAKE3 known answers of
+       libs/hash/src/root.zig:121-127 ("hello world") and :173-247 (45
+       lengths 0..4097 of a repeating 16-byte pattern).
 2. Python-int vectors computed from definitions (tests/golden/pyref.py), with
    no use of the oracle or the HIP library:
      * ``fr_constants.json``  -- Montgomery constants, roots, two-adicity, QNR;
@@ -93,6 +96,23 @@ def extract_ref_field_vectors() -> None:
         assert len(vecs) == 9
         out[name] = vecs
     dump("ref_field_vectors.json", {"source": "libs/field/src/lib.zig:257-276", "vectors": out})
+
+
+def extract_ref_blake3() -> None:
+    path = os.path.join(REF, "libs/hash/src/root.zig")
+    text = open(path).read().splitlines()
+    hw = []
+    for line in text[121:127]:  # root.zig:122-127
+        hw += re.findall(r"0x([0-9a-f]{2})", line)
+    assert len(hw) == 32, len(hw)
+    body = "\n".join(text[172:247])  # root.zig:173-247
+    pat = re.search(r"const pattern = \[_\]u8\{([^}]*)\}", body)
+    pattern = [int(x) for x in pat.group(1).split(",")]
+    vecs = [{"len": int(a), "digest": b}
+            for a, b in re.findall(r'\.len = (\d+), \.digest = "([0-9a-f]{64})"', body)]
+    assert len(pattern) == 16 and len(vecs) == 45, (len(pattern), len(vecs))
+    dump("ref_blake3_kat.json", {"source": "libs/hash/src/root.zig:121-127, 173-247",
+                                 "hello_world": "".join(hw), "pattern": pattern, "vectors": vecs})
 
 
 def lcg(seed: int):
@@ -198,9 +218,11 @@ def textbook_digests() -> None:
 
 
 def main() -> None:
-    steps = sys.argv[1:] or ["kat", "field", "const", "frvec", "dft", "textbook"]
+    steps = sys.argv[1:] or ["kat", "field", "blake3", "const", "frvec", "dft", "textbook"]
     if "kat" in steps:
         extract_ref_kat()
+    if "blake3" in steps:
+        extract_ref_blake3()
     if "field" in steps:
         extract_ref_field_vectors()
     if "const" in steps:

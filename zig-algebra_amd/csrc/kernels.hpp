@@ -77,6 +77,18 @@ hipError_t launch_bit_reverse(int field, int L, uint32_t *data, hipStream_t stre
 // the Montgomery product of two data-form values is x*y*2^251 (R' = 2^261),
 // which a later scale by c*2^5 (dev form) returns to the 2^256 form.
 hipError_t launch_pointwise_mul(int field, uint32_t *a, const uint32_t *b, size_t n, hipStream_t stream);
+// Merkle trees (merkle_kernels.hip).  heap is the node array (32 bytes a
+// node, node s + i of the level starting at s); with `full` it receives the
+// whole tree (2^(cap_log + 1) nodes), otherwise only the nodes above the
+// first launch, 2^merkle_heap_log(cap_log, false) of them.  Node 1 is the root.
+int merkle_heap_log(int cap_log, bool full);
+// FRI layer of 2^(leaf_log + 1) user-layout values: leaf j hashes values j
+// and j + 2^leaf_log.
+hipError_t launch_merkle_layer(int field, const void *evals, int leaf_log, uint32_t *heap, bool full,
+                               hipStream_t s);
+// Tree over `count` stored hashes padded to 2^cap_log leaves.
+hipError_t launch_merkle_from_hashes(const void *hashes, uint64_t count, int cap_log, uint32_t *heap, bool full,
+                                     hipStream_t s);
 // Elementwise device field op (0 add, 1 sub, 2 mul) on user-layout arrays.
 hipError_t launch_field_op(int field, int op, size_t n, const uint32_t *a, const uint32_t *b, uint32_t *out,
                            hipStream_t stream);

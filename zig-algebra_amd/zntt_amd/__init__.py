@@ -97,7 +97,7 @@ EXPORTED = [
     "zntt_server_create", "zntt_server_submit", "zntt_server_query", "zntt_server_wait", "zntt_server_cancel",
     "zntt_server_read", "zntt_server_release", "zntt_server_drain", "zntt_server_stats_get", "zntt_server_sim_pump",
     "zntt_server_sim_complete", "zntt_server_destroy", "zntt_fill_counter",
-    "zntt_poly_mul", "zntt_poly_mul_device",
+    "zntt_poly_mul", "zntt_poly_mul_device", "zntt_merkle_commit", "zntt_merkle_from_hashes",
 ]
 
 REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED, REQ_CANCELLED = 0, 1, 2, 3, 4
@@ -204,6 +204,8 @@ def _load():
     L.zntt_fill_counter.argtypes = [vp, i32, u64, u64, vp, sz, vp]
     L.zntt_poly_mul.argtypes = [vp, i32, vp, sz, vp, sz, vp, sz, ctypes.POINTER(sz)]
     L.zntt_poly_mul_device.argtypes = [vp, i32, vp, sz, vp, sz, vp, vp]
+    L.zntt_merkle_commit.argtypes = [vp, i32, vp, sz, vp, vp, vp]
+    L.zntt_merkle_from_hashes.argtypes = [vp, vp, sz, vp, vp, vp]
     return L
 
 
@@ -352,6 +354,25 @@ class Context:
         if stream is None:
             stream = lib.zntt_ctx_stream(self._h)
         check(lib.zntt_poly_mul_device(self._h, field, a_ptr, a_len, b_ptr, b_len, out_ptr, stream))
+
+    def merkle_commit_device(self, field: int, evals_ptr: int, log_n: int, nodes_ptr: int | None = None,
+                             stream: int | None = None) -> bytes:
+        """Root of the FRI layer commitment of 2^log_n device evaluations;
+        with nodes_ptr the whole tree (2^log_n nodes of 32 bytes) lands there."""
+        if stream is None:
+            stream = lib.zntt_ctx_stream(self._h)
+        root = (ctypes.c_uint8 * 32)()
+        check(lib.zntt_merkle_commit(self._h, field, evals_ptr, log_n, nodes_ptr, root, stream))
+        return bytes(root)
+
+    def merkle_from_hashes_device(self, hashes_ptr: int, count: int, nodes_ptr: int | None = None,
+                                  stream: int | None = None) -> bytes:
+        """Root of MerkleTree.initFromHashes over `count` device leaf hashes."""
+        if stream is None:
+            stream = lib.zntt_ctx_stream(self._h)
+        root = (ctypes.c_uint8 * 32)()
+        check(lib.zntt_merkle_from_hashes(self._h, hashes_ptr, count, nodes_ptr, root, stream))
+        return bytes(root)
 
     def lde(self, field: int, coeffs: np.ndarray, log_k: int, log_blowup: int, shift) -> np.ndarray:
         out = empty(field, 1 << (log_k + log_blowup))
