@@ -20,14 +20,16 @@
 // cost: about n compressions per layer of n values against 2 to 32 bytes of
 // HBM traffic per value, so the kernel is VALU-bound.
 //
-// Layout of the work: one workgroup of 256 threads owns 2^levels consecutive
-// children (up to 512) of the level it starts from.  Each thread forms two
-// children (leaf hashes, or stored nodes) and their parent in registers; the
-// remaining levels reduce through LDS, so one launch climbs up to 9 levels and
-// leaves one node per workgroup.  A layer of 2^23 leaves takes three launches
-// (2^23 -> 2^14 -> 2^5 -> 1).  Nodes are written to the caller's heap when it
-// asks for the whole tree; otherwise only each workgroup's top node is kept, in
-// a small heap of 2 * 2^(k - 1 - levels) nodes that the later launches finish.
+// Layout of the work: one workgroup of 256 threads owns up to 512 consecutive
+// children of the level it starts from.  Each thread forms two children (leaf
+// hashes, or stored nodes) and their parent in registers; further levels
+// reduce through LDS.  A launch over a level larger than 512 nodes climbs
+// three levels (512 -> 64 nodes per workgroup, whole waves busy at every
+// level); the launch over the last 512 or fewer climbs to the root, so a
+// layer of 2^23 leaves takes six launches (2^23 -> 2^20 -> 2^17 -> 2^14 ->
+// 2^11 -> 2^8 -> 1).  Nodes are written to the caller's heap when it asks for
+// the whole tree; otherwise only the first launch's top level and what lies
+// above it are kept, in a heap of 2 * 2^(k - 1 - 3) nodes.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -113,21 +115,23 @@ __device__ __forceinline__ void store8(uint32_t *p, const uint32_t w[8]) {
 }
 
 // toBytes of a BN254 Fr element stored in the reference's Montgomery form
-// x * 2^256: mont(x * 2^256, 32) = x * 2^261 / 2^261 = x with R' = 2^261,
-// then the canonical representative repacked into 8 words.
-__device__ __forceinline__ void fr_canonical_words(const uint32_t *user, size_t i, uint32_t w[8]) {
-  Fr9 x = FrT::load_user(user, i);
+// x * 2^256: mont(x * 2^256, 32) = x * 2^261 / 2^261 = x with R' = 2^261.
+// The product is below (2^256 * 32) / 2^261 + r = r + 1, so one conditional
+// subtraction of r leaves the canonical representative (r itself only for a
+// stored value of r, the unreduced zero).
+__device__ __forceinline__ void fr_canonical_words(const Fr9 &mont, uint32_t w[8]) {
   Fr9 k;
 #pragma unroll
   for (int q = 0; q < 9; q++) k.v[q] = 0;
   k.v[0] = 32;
-  x = fr_mul(x, k);
-  fr_canon(x);
+  Fr9 x = fr_mul(mont, k);
+  fr_csub_r(x);
   fr_pack8(x, w);
 }
 
 // Children of the first launch over a FRI layer: leaf j of the pairs
-// (j, j + half), half = the leaf count.
+// (j, j + half), half = the leaf count.  get2 forms leaves j and j + 1 (the
+// two children of one parent) with every load issued before any hashing.
 template <class F>
 struct LayerLeaves;
 
@@ -137,9 +141,20 @@ struct LayerLeaves<FrT> {
   uint64_t half;
   __device__ void get(uint64_t j, uint32_t h[8]) const {
     uint32_t m[16];
-    fr_canonical_words(evals, j, m);
-    fr_canonical_words(evals, j + half, m + 8);
+    fr_canonical_words(FrT::load_user(evals, j), m);
+    fr_canonical_words(FrT::load_user(evals, j + half), m + 8);
     hash_block(m, 64, h);
+  }
+  __device__ void get2(uint64_t j, uint32_t (&out)[16]) const {
+    const Fr9 a0 = FrT::load_user(evals, j), b0 = FrT::load_user(evals, j + half);
+    const Fr9 a1 = FrT::load_user(evals, j + 1), b1 = FrT::load_user(evals, j + 1 + half);
+    uint32_t m[16];
+    fr_canonical_words(a0, m);
+    fr_canonical_words(b0, m + 8);
+    hash_block(m, 64, out);
+    fr_canonical_words(a1, m);
+    fr_canonical_words(b1, m + 8);
+    hash_block(m, 64, out + 8);
   }
 };
 
@@ -147,11 +162,19 @@ template <>
 struct LayerLeaves<GlT> {
   const uint32_t *evals;
   uint64_t half;
-  __device__ void get(uint64_t j, uint32_t h[8]) const {
-    const uint2 a = reinterpret_cast<const uint2 *>(evals)[j];
-    const uint2 b = reinterpret_cast<const uint2 *>(evals)[j + half];
+  __device__ static void leaf(uint2 a, uint2 b, uint32_t h[8]) {
     const uint32_t m[16] = {a.x, a.y, b.x, b.y, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     hash_block(m, 16, h);
+  }
+  __device__ void get(uint64_t j, uint32_t h[8]) const {
+    const uint2 *e = reinterpret_cast<const uint2 *>(evals);
+    leaf(e[j], e[j + half], h);
+  }
+  __device__ void get2(uint64_t j, uint32_t (&out)[16]) const {
+    const uint4 a = *reinterpret_cast<const uint4 *>(evals + 2 * j);
+    const uint4 b = *reinterpret_cast<const uint4 *>(evals + 2 * (j + half));
+    leaf(make_uint2(a.x, a.y), make_uint2(b.x, b.y), out);
+    leaf(make_uint2(a.z, a.w), make_uint2(b.z, b.w), out + 8);
   }
 };
 
@@ -159,10 +182,16 @@ struct LayerLeaves<GlT> {
 struct LayerLeaves31 {
   const uint32_t *evals;
   uint64_t half;
-  __device__ void get(uint64_t j, uint32_t h[8]) const {
-    const uint32_t a = evals[2 * j], b = evals[2 * (j + half)];
+  __device__ static void leaf(uint32_t a, uint32_t b, uint32_t h[8]) {
     const uint32_t m[16] = {a, b, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     hash_block(m, 8, h);
+  }
+  __device__ void get(uint64_t j, uint32_t h[8]) const { leaf(evals[2 * j], evals[2 * (j + half)], h); }
+  __device__ void get2(uint64_t j, uint32_t (&out)[16]) const {
+    const uint4 a = *reinterpret_cast<const uint4 *>(evals + 2 * j);
+    const uint4 b = *reinterpret_cast<const uint4 *>(evals + 2 * (j + half));
+    leaf(a.x, b.x, out);
+    leaf(a.z, b.z, out + 8);
   }
 };
 
@@ -181,16 +210,22 @@ struct StoredNodes {
       for (int i = 0; i < 8; i++) h[i] = kEmpty[i];
     }
   }
+  __device__ void get2(uint64_t j, uint32_t (&out)[16]) const {
+    get(j, out);
+    get(j + 1, out + 8);
+  }
 };
 
-// One launch: each workgroup reduces 2^levels consecutive children of a
-// level of 2^child_log nodes to one node.  heap is the reference's node array
-// (node s + i at heap + 8 * (s + i)); children are written at their heap
-// positions when write_children, inner nodes when write_inner, and each
-// workgroup's top node always.  levels == 0: the single child is the root.
+// One launch over a level of 2^child_log nodes: each workgroup owns 2^span_log
+// consecutive children and climbs `levels` levels above them, ending with
+// 2^(span_log - levels) nodes.  heap is the reference's node array (node s + i
+// at heap + 8 * (s + i)); children are written at their heap positions when
+// write_children, inner nodes when write_inner, and the nodes of the last
+// level climbed always.  levels == 0: the single child is the root.
 template <class Src>
-__global__ __launch_bounds__(kMerkleBlock) void merkle_kernel(Src src, uint32_t *heap, int child_log, int levels,
-                                                              int write_children, int write_inner, int zero_node0) {
+__global__ __launch_bounds__(kMerkleBlock) void merkle_kernel(Src src, uint32_t *heap, int child_log, int span_log,
+                                                              int levels, int write_children, int write_inner,
+                                                              int zero_node0) {
   __shared__ uint32_t lds[8 * kMerkleBlock];
   const uint32_t t = threadIdx.x;
   const uint64_t C = 1ull << child_log;
@@ -203,14 +238,13 @@ __global__ __launch_bounds__(kMerkleBlock) void merkle_kernel(Src src, uint32_t 
     }
     return;
   }
-  const uint64_t base = (uint64_t)blockIdx.x << levels;  // first child of this workgroup
-  uint32_t cnt = 1u << (levels - 1);                       // nodes one level up
+  const uint64_t base = (uint64_t)blockIdx.x << span_log;  // first child of this workgroup
+  uint32_t cnt = 1u << (span_log - 1);                       // nodes one level up
   uint32_t h[8];
   if (t < cnt) {
     uint32_t m[16];
     const uint64_t j = base + 2 * t;
-    src.get(j, m);
-    src.get(j + 1, m + 8);
+    src.get2(j, m);
     if (write_children) {
       store8(heap + 8 * (C + j), m);
       store8(heap + 8 * (C + j + 1), m + 8);
@@ -240,20 +274,29 @@ __global__ __launch_bounds__(kMerkleBlock) void merkle_kernel(Src src, uint32_t 
 }
 
 template <class Src>
-hipError_t launch_level(const Src &src, uint32_t *heap, int child_log, int levels, bool write_children,
-                        bool write_inner, bool zero_node0, hipStream_t s) {
-  const unsigned grid = (unsigned)(1ull << (child_log - levels));
-  hipLaunchKernelGGL(merkle_kernel<Src>, dim3(grid), dim3(kMerkleBlock), 0, s, src, heap, child_log, levels,
-                     (int)write_children, (int)write_inner, (int)zero_node0);
+hipError_t launch_level(const Src &src, uint32_t *heap, int child_log, int span_log, int levels,
+                        bool write_children, bool write_inner, bool zero_node0, hipStream_t s) {
+  const unsigned grid = (unsigned)(1ull << (child_log - span_log));
+  hipLaunchKernelGGL(merkle_kernel<Src>, dim3(grid), dim3(kMerkleBlock), 0, s, src, heap, child_log, span_log,
+                     levels, (int)write_children, (int)write_inner, (int)zero_node0);
   return hipGetLastError();
 }
+
+// Levels one launch climbs over a level of 2^log nodes.  A level that one
+// workgroup covers is finished by it.  A larger one stops where each
+// workgroup still has 64 nodes (512 -> 256 -> 128 -> 64): every level climbed
+// fills whole waves, and the next launch starts from an eighth of the nodes.
+// Climbing on to one node per workgroup would run six levels on waves with
+// 32, 16, ... 1 busy lanes.
+int climb_levels(int log) { return log <= kMerkleMaxLevels ? log : 3; }
 
 // Finishes a tree whose level of 2^top_log nodes is already in heap.
 hipError_t finish_tree(uint32_t *heap, int top_log, hipStream_t s) {
   while (top_log > 0) {
-    const int lv = top_log < kMerkleMaxLevels ? top_log : kMerkleMaxLevels;
+    const int span = top_log < kMerkleMaxLevels ? top_log : kMerkleMaxLevels;
+    const int lv = climb_levels(top_log);
     StoredNodes src{heap + 8 * (1ull << top_log), 1ull << top_log};
-    hipError_t e = launch_level(src, heap, top_log, lv, false, true, top_log == lv, s);
+    hipError_t e = launch_level(src, heap, top_log, span, lv, false, true, top_log == lv, s);
     if (e != hipSuccess) return e;
     top_log -= lv;
   }
@@ -262,19 +305,16 @@ hipError_t finish_tree(uint32_t *heap, int top_log, hipStream_t s) {
 
 template <class Src>
 hipError_t build_tree(const Src &src, int cap_log, uint32_t *heap, bool full, hipStream_t s) {
-  const int lv = cap_log < kMerkleMaxLevels ? cap_log : kMerkleMaxLevels;
-  hipError_t e = launch_level(src, heap, cap_log, lv, full, full, cap_log == lv, s);
+  const int span = cap_log < kMerkleMaxLevels ? cap_log : kMerkleMaxLevels;
+  const int lv = climb_levels(cap_log);
+  hipError_t e = launch_level(src, heap, cap_log, span, lv, full, full, cap_log == lv, s);
   if (e != hipSuccess) return e;
   return finish_tree(heap, cap_log - lv, s);
 }
 
 }  // namespace
 
-int merkle_heap_log(int cap_log, bool full) {
-  if (full) return cap_log + 1;
-  const int lv = cap_log < kMerkleMaxLevels ? cap_log : kMerkleMaxLevels;
-  return cap_log - lv + 1;
-}
+int merkle_heap_log(int cap_log, bool full) { return full ? cap_log + 1 : cap_log - climb_levels(cap_log) + 1; }
 
 hipError_t launch_merkle_layer(int field, const void *evals, int leaf_log, uint32_t *heap, bool full,
                                hipStream_t s) {
