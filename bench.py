@@ -482,6 +482,71 @@ def run_merkle(args, Z, ctx):
     return line
 
 
+def run_fri(args, Z, ctx):
+    """FRI prove (libs/fri/src/root.zig:311-441) over 2^log_n BN254 Fr
+    evaluations of a degree < 2^(log_n-3) polynomial resident in HBM: rounds
+    down to a 2^8 final domain, 2^5 residual coefficients, 80 queries; the
+    library's own transcript."""
+    import numpy as np
+    import torch
+    F, L = Z.FIELD_FR, args.log_n
+    n = 1 << L
+    cfg = (L, L - 3, 8, 5, 80)
+    stream = torch.cuda.current_stream()
+    sp = stream.cuda_stream
+    ev = torch.zeros(n * 4, dtype=torch.int64, device="cuda")
+    Z.fill_counter_device(ctx, F, 3, 9, ev.data_ptr(), 1 << (L - 3), stream=sp)
+    ctx.execute_device(F, Z.FORWARD, L, ev.data_ptr(), Z.primitive_root(F, L), stream=sp)
+    torch.cuda.synchronize()
+    proofs = []
+
+    def step():
+        proofs.append(ctx.fri_prove(F, Z.Transcript(b"bench"), ev.data_ptr(), n, *cfg, stream=sp))
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    same = all(p == proofs[0] for p in proofs)
+    rounds = L - 8
+    compressions = sum((1 << (L - r)) - 1 for r in range(rounds))
+    achieved = compressions * BLAKE3_OPS / (ms * 1e-3)
+    roofline = {"bound": "valu", "achieved": achieved / 1e12, "peak": VALU_MUL_PEAK / 1e12, "unit": "Tvalu32/s",
+                "frac": achieved / VALU_MUL_PEAK, "traffic": None}
+    cpu, ok = None, None
+    if not args.skip_cpu:
+        import oracle as O
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from fri_verify import verify
+        ok = verify(F, b"bench", proofs[-1], *cfg)  # the restated verifier, on the oracle's BLAKE3
+        m = 20
+        cev = O.gen_uniform(O.FIELD_FR, 5, 1 << m)
+        t1 = time.time()
+        st, _ = O.fri_prove(O.FIELD_FR, O.Transcript(b"bench"), cev, m, m - 3, 8, 5, 80)
+        cs = time.time() - t1
+        assert st == O.OK
+        cpu = {"value": (1 << m) / cs / 1e6, "unit": "Melem/s", "cores": 1, "kind": "port",
+               "sample": f"prove over 2^{m} BN254 Fr evaluations (same config shape), C oracle restating "
+                         f"proveOn with its per-element inverse in the fold, 1 thread, {cs:.1f} s"}
+    line = base_line(f"Melem/s BN254 Fr FRI prove 2^{L} (commit, fold, residual, 80 queries)", n / (ms * 1e-3) / 1e6,
+                     "Melem/s", args, ms,
+                     {"workload": f"FRI prove over 2^{L} BN254 Fr evaluations resident in HBM "
+                                  f"(degree < 2^{L - 3}), {rounds} rounds to a 2^8 final domain, "
+                                  "2^5 residual coefficients, 80 queries, library transcript",
+                      "field": "bn254_fr", "evaluations": n},
+                     "synthetic: counter-stream coefficients (seed 3), evaluated on the domain by the library's NTT",
+                     "u32")
+    line.update({"roofline": roofline, "cpu_baseline": cpu,
+                 "extras": {"proof_verifies": ok, "proofs_identical": same, "rounds": rounds,
+                            "timing": "host wall clock around each call (the prover waits for each round's "
+                                      "root on the host)"}})
+    return line
+
+
 def serve_muls(arrivals):
     """Algorithmic 32-bit multiplies of a schedule: per request (N/2) log2 N
     butterfly products plus 2^k coset products, N = 2^(k+b)."""
@@ -569,7 +634,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="fr-ntt",
                     choices=["fr-ntt", "gl-batch", "bb-batch", "kb-batch", "fr-uneven", "serve", "ref-2p20",
-                             "poly-mul", "merkle"])
+                             "poly-mul", "merkle", "fri"])
     ap.add_argument("--log-n", type=int, default=24)
     ap.add_argument("--cpu-log-n", type=int, default=24, help="size of the CPU-baseline sample")
     ap.add_argument("--skip-cpu", action="store_true")
@@ -601,6 +666,8 @@ def main() -> int:
         line = run_small_batch(args, Z, ctx, Z.FIELD_KB)
     elif args.workload == "merkle":
         line = run_merkle(args, Z, ctx)
+    elif args.workload == "fri":
+        line = run_fri(args, Z, ctx)
     else:
         line = run_serve(args, Z, ctx)
     print(json.dumps(line), flush=True)

@@ -67,7 +67,8 @@ enum zntt_status {
   ZNTT_INVALID_ARGUMENT = 8,  /* null context/pointer, unknown field id          */
   ZNTT_UNSUPPORTED = 9,       /* a size this device path does not implement      */
   ZNTT_ORDER_TOO_LARGE = 10,  /* error.OrderTooLarge    roots.zig:150 (LDE/roots) */
-  ZNTT_DEGREE_TOO_LARGE = 11  /* error.DegreeTooLarge   libs/poly/src/poly.zig:200 */
+  ZNTT_DEGREE_TOO_LARGE = 11, /* error.DegreeTooLarge   libs/poly/src/poly.zig:200 */
+  ZNTT_INVALID_PARAMETERS = 12 /* FriError.InvalidParameters libs/fri/src/root.zig:77 */
 };
 
 enum zntt_field {
@@ -386,6 +387,82 @@ int zntt_merkle_commit(zntt_ctx *ctx, int field, const void *evals, size_t log_n
  * cap = count rounded up to a power of two (at most 2^ZNTT_MAX_LOG_N).  The
  * root goes to host memory `root` as above (merkle_tree.zig:147-182, :189). */
 int zntt_merkle_from_hashes(zntt_ctx *ctx, const void *hashes, size_t count, void *nodes, void *root, void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* FRI prover: the LDE's consumer (libs/fri/src/root.zig:311-441, the query  */
+/* openings :506-573).  Each round commits the current layer (the Merkle     */
+/* commitment above), absorbs the root, squeezes alpha with                  */
+/* challengeFieldChecked and folds on the device; the last layer is          */
+/* interpolated by an inverse transform on the device, its first             */
+/* 2^log_residual_degree coefficients absorbed; then num_queries indices are */
+/* drawn with challengeU64 and the pair values and Merkle paths gathered     */
+/* from the device.  The transcript stays the caller's: the library calls    */
+/* back into it, so a Zig shim passes its own zig-transcript object.         */
+/* ------------------------------------------------------------------------ */
+
+/* Config (root.zig:174-203). */
+typedef struct {
+  size_t log_domain;
+  size_t log_initial_degree;
+  size_t log_final;
+  size_t log_residual_degree;
+  size_t num_queries;
+} zntt_fri_config;
+
+/* A transcript as the prover uses it: absorbBytes and challengeBytes
+ * (libs/transcript/src/root.zig:55-60, :93-114).  challengeU64 and
+ * challengeFieldChecked are built from challenge_bytes as the reference
+ * builds them (:117-121, :167-180). */
+typedef struct {
+  void (*absorb_bytes)(void *transcript, const uint8_t *data, size_t len);
+  void (*challenge_bytes)(void *transcript, uint8_t *out, size_t len);
+} zntt_transcript_ops;
+
+/* The Proof (root.zig:207-245) in caller-allocated host arrays, sized by
+ * zntt_fri_proof_sizes (R = log_domain - log_final rounds, B = the field's
+ * toBytes length: 32 Fr, 8 Goldilocks, 4 BabyBear / KoalaBear):
+ *   roots       R x 32          layers[r].merkle_root (log_size = log_domain - r)
+ *   residual    2^log_residual_degree x B   the residual's serializeElem bytes
+ *   pair_index  num_queries     queries[q].pair_index
+ *   values      num_queries x R x 2B        queries[q].values[r] (serializePair)
+ *   paths       num_queries x sum_r (log_domain - r - 1) x 32, query by query,
+ *               round by round, each path leaf-sibling first (MerkleTree.prove). */
+typedef struct {
+  uint8_t *roots;
+  uint8_t *residual;
+  uint64_t *pair_index;
+  uint8_t *values;
+  uint8_t *paths;
+} zntt_fri_proof;
+
+int zntt_fri_proof_sizes(int field, const zntt_fri_config *cfg, size_t *roots_bytes, size_t *residual_bytes,
+                         size_t *values_bytes, size_t *paths_bytes);
+
+/* prove(F, allocator, transcript, evals, config) -- root.zig:311-332.
+ * `evals`: device buffer of evals_len values (storage layout, natural domain
+ * order, 16-byte aligned).  Errors, in the reference's order:
+ * ZNTT_INVALID_PARAMETERS for an invalid config, evals_len != 2^log_domain or
+ * log_domain above the field's two-adicity; ZNTT_UNSUPPORTED beyond
+ * ZNTT_MAX_LOG_N.  Work runs on `stream`; the call returns when the proof is
+ * written.  Device memory: about 2^log_domain x (value + 32) bytes for the
+ * folded layers and the trees, freed before returning. */
+int zntt_fri_prove(zntt_ctx *ctx, int field, const zntt_fri_config *cfg, const zntt_transcript_ops *ops,
+                   void *transcript, const void *evals, size_t evals_len, zntt_fri_proof *proof, void *stream);
+
+/* The library's own transcript, byte for byte zig-transcript's
+ * (Transcript.init / absorbBytes / challengeBytes).  Its two functions match
+ * zntt_transcript_ops, so {zntt_transcript_absorb_bytes,
+ * zntt_transcript_challenge_bytes} with a zntt_transcript* serves as the
+ * prover's transcript. */
+typedef struct zntt_transcript zntt_transcript;
+int zntt_transcript_create(const void *label, size_t len, zntt_transcript **out);
+void zntt_transcript_destroy(zntt_transcript *t);
+void zntt_transcript_absorb_bytes(void *t, const uint8_t *data, size_t len);
+void zntt_transcript_challenge_bytes(void *t, uint8_t *out, size_t len);
+
+/* BLAKE3 with a 32-byte output (the hash of the transcript and the trees),
+ * exposed for checks. */
+void zntt_blake3(const void *data, size_t len, uint8_t *out);
 
 /* Blocks until every call enqueued on the context's own stream is done. */
 int zntt_ctx_synchronize(zntt_ctx *ctx);

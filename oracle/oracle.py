@@ -41,6 +41,8 @@ OUT_OF_MEMORY = 5
 INVALID_ARGUMENT = 8
 ORDER_TOO_LARGE = 10
 DEGREE_TOO_LARGE = 11
+FRI_INVALID_PARAMETERS = 12
+NUM_BYTES = {FIELD_FR: 32, FIELD_GL: 8, FIELD_BB: 4, FIELD_KB: 4}
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "liboracle.so")
@@ -86,6 +88,18 @@ def lib():
         L.oracle_fri_leaves.argtypes = [i32, vp, sz, vp]
         L.oracle_merkle_from_hashes.argtypes = [vp, sz, vp, vp]
         L.oracle_merkle_commit.argtypes = [i32, vp, sz, vp, vp]
+        L.oracle_tr_new.argtypes = [ctypes.c_char_p, sz]
+        L.oracle_tr_new.restype = vp
+        L.oracle_tr_free.argtypes = [vp]
+        L.oracle_tr_free.restype = None
+        L.oracle_tr_absorb.argtypes = [vp, ctypes.c_char_p, sz]
+        L.oracle_tr_absorb.restype = None
+        L.oracle_tr_challenge_bytes.argtypes = [vp, vp, sz]
+        L.oracle_tr_challenge_bytes.restype = None
+        L.oracle_tr_challenge_u64.argtypes = [vp]
+        L.oracle_tr_challenge_u64.restype = u64
+        L.oracle_tr_challenge_field.argtypes = [vp, i32, vp]
+        L.oracle_fri_prove.argtypes = [i32, vp, vp, sz, sz, sz, sz, sz, sz, vp, vp, vp, vp, vp]
         L.oracle_init()
         _lib = L
     return _lib
@@ -308,3 +322,66 @@ def merkle_commit(field: int, evals: np.ndarray) -> tuple[np.ndarray, bytes]:
     st = lib().oracle_merkle_commit(field, _ptr(np.ascontiguousarray(evals)), n, _bptr(nodes), _bptr(root))
     assert st == OK, st
     return nodes, root.tobytes()
+
+
+class Transcript:
+    """The oracle's Fiat-Shamir transcript (libs/transcript/src/root.zig)."""
+
+    def __init__(self, label: bytes):
+        self._h = lib().oracle_tr_new(bytes(label), len(label))
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().oracle_tr_free(self._h)
+            self._h = None
+
+    def absorb_bytes(self, data: bytes) -> None:
+        lib().oracle_tr_absorb(self._h, bytes(data), len(data))
+
+    def absorb_u64(self, v: int) -> None:
+        self.absorb_bytes(int(v).to_bytes(8, "little"))
+
+    def challenge_bytes(self, n: int) -> bytes:
+        out = np.zeros(max(n, 1), dtype=np.uint8)
+        lib().oracle_tr_challenge_bytes(self._h, _bptr(out), n)
+        return out[:n].tobytes()
+
+    def challenge_u64(self) -> int:
+        return int(lib().oracle_tr_challenge_u64(self._h))
+
+    def challenge_field(self, field: int) -> np.ndarray:
+        out = empty(field, 1)
+        st = lib().oracle_tr_challenge_field(self._h, field, _ptr(out))
+        assert st == OK
+        return out[0]
+
+
+def fri_proof_sizes(field: int, log_domain: int, log_final: int, log_residual_degree: int, num_queries: int):
+    rounds = log_domain - log_final
+    depth = sum(log_domain - r - 1 for r in range(rounds))
+    nb = NUM_BYTES[field]
+    return {"rounds": rounds, "roots": rounds * 32, "residual": (1 << log_residual_degree) * nb,
+            "values": num_queries * rounds * 2 * nb, "paths": num_queries * depth * 32}
+
+
+def fri_prove(field: int, transcript: Transcript, evals: np.ndarray, log_domain: int, log_initial_degree: int,
+              log_final: int, log_residual_degree: int, num_queries: int):
+    """prove(F, transcript, evals, config) -> (status, proof dict of flat arrays)."""
+    rounds = max(log_domain - log_final, 0)
+    sz = fri_proof_sizes(field, log_domain, log_final, log_residual_degree, num_queries) if rounds > 0 else None
+    if sz is None:
+        sz = {"roots": 0, "residual": 0, "values": 0, "paths": 0}
+    roots = np.zeros(max(sz["roots"], 1), dtype=np.uint8)
+    residual = np.zeros(max(sz["residual"], 1), dtype=np.uint8)
+    pair = np.zeros(max(num_queries, 1), dtype=np.uint64)
+    values = np.zeros(max(sz["values"], 1), dtype=np.uint8)
+    paths = np.zeros(max(sz["paths"], 1), dtype=np.uint8)
+    ev = np.ascontiguousarray(evals)
+    st = lib().oracle_fri_prove(field, transcript._h, _ptr(ev), ev.shape[0], log_domain, log_initial_degree,
+                                log_final, log_residual_degree, num_queries, _bptr(roots), _bptr(residual),
+                                _ptr(pair), _bptr(values), _bptr(paths))
+    if st != OK:
+        return st, None
+    return st, {"roots": roots[:sz["roots"]].tobytes(), "residual": residual[:sz["residual"]].tobytes(),
+                "pair_index": [int(x) for x in pair[:num_queries]], "values": values[:sz["values"]].tobytes(),
+                "paths": paths[:sz["paths"]].tobytes()}

@@ -1178,3 +1178,299 @@ int oracle_merkle_commit(int field, const void *evals, size_t n, uint8_t *nodes,
   free(leaves);
   return st;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Fiat-Shamir transcript (libs/transcript/src/root.zig:38-180).             */
+/*                                                                           */
+/* The transcript is a BLAKE3 hasher fed a byte stream: init writes the      */
+/* label's length (u64 little-endian) and the label, absorbBytes the data's  */
+/* length and the data.  A squeeze finalises the hash of everything fed since */
+/* the last re-key, hands out its first bytes (re-hashing the digest for     */
+/* each further 32), and re-keys: the stream restarts as the last digest.    */
+/* So the state is exactly the bytes fed since the last re-key, which is how */
+/* this restatement keeps it.                                                */
+/* ------------------------------------------------------------------------ */
+
+typedef struct { uint8_t *buf; size_t len, cap; } oracle_tr;
+
+static void tr_feed(oracle_tr *t, const void *p, size_t n) {
+  if (t->len + n > t->cap) {
+    size_t cap = t->cap ? t->cap : 256;
+    while (cap < t->len + n) cap *= 2;
+    t->buf = (uint8_t *)realloc(t->buf, cap);
+    t->cap = cap;
+  }
+  if (n) memcpy(t->buf + t->len, p, n);
+  t->len += n;
+}
+
+static void tr_feed_len(oracle_tr *t, uint64_t n) {
+  uint8_t b[8];
+  for (int i = 0; i < 8; i++) b[i] = (uint8_t)(n >> (8 * i));
+  tr_feed(t, b, 8);
+}
+
+/* Transcript.init(label) */
+oracle_tr *oracle_tr_new(const uint8_t *label, size_t len) {
+  oracle_tr *t = (oracle_tr *)calloc(1, sizeof *t);
+  tr_feed_len(t, len);
+  tr_feed(t, label, len);
+  return t;
+}
+
+void oracle_tr_free(oracle_tr *t) {
+  if (!t) return;
+  free(t->buf);
+  free(t);
+}
+
+/* absorbBytes */
+void oracle_tr_absorb(oracle_tr *t, const uint8_t *data, size_t len) {
+  tr_feed_len(t, len);
+  tr_feed(t, data, len);
+}
+
+/* challengeBytes */
+void oracle_tr_challenge_bytes(oracle_tr *t, uint8_t *out, size_t n) {
+  uint8_t d[32];
+  oracle_blake3(t->buf ? t->buf : (const uint8_t *)"", t->len, d);
+  size_t off = 0;
+  while (off < n) {
+    const size_t k = n - off < 32 ? n - off : 32;
+    memcpy(out + off, d, k);
+    off += k;
+    if (off < n) oracle_blake3(d, 32, d);
+  }
+  t->len = 0;
+  tr_feed(t, d, 32);
+}
+
+/* challengeU64 */
+uint64_t oracle_tr_challenge_u64(oracle_tr *t) {
+  uint8_t b[8];
+  oracle_tr_challenge_bytes(t, b, 8);
+  uint64_t v = 0;
+  for (int i = 0; i < 8; i++) v |= (uint64_t)b[i] << (8 * i);
+  return v;
+}
+
+/* challengeFieldChecked: draw NUM_BYTES little-endian bytes until they encode
+ * a value below the modulus (fromBytesChecked, field.zig:217-221 and
+ * :975-979); out receives the element in the storage layout. */
+int oracle_tr_challenge_field(oracle_tr *t, int field, void *out) {
+  fr_init();
+  if (field == FIELD_FR) {
+    for (;;) {
+      uint8_t b[32];
+      oracle_tr_challenge_bytes(t, b, 32);
+      fr_t v;
+      for (int k = 0; k < 4; k++) {
+        v.l[k] = 0;
+        for (int i = 0; i < 8; i++) v.l[k] |= (uint64_t)b[8 * k + i] << (8 * i);
+      }
+      if (fr_lt(v.l, FR_MOD)) {
+        v = fr_to_mont(v);
+        memcpy(out, &v, 32);
+        return OR_OK;
+      }
+    }
+  }
+  sf_ctx c;
+  if (!sf_for(field, &c)) return OR_INVALID_ARGUMENT;
+  const int nb = (c.bits + 7) / 8;
+  for (;;) {
+    uint8_t b[8];
+    oracle_tr_challenge_bytes(t, b, (size_t)nb);
+    uint64_t v = 0;
+    for (int i = 0; i < nb; i++) v |= (uint64_t)b[i] << (8 * i);
+    if (v < c.p) {
+      memcpy(out, &v, 8);
+      return OR_OK;
+    }
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* FRI prover (libs/fri/src/root.zig:311-441, buildQueries :506-573).        */
+/* ------------------------------------------------------------------------ */
+
+/* One element of any field, with the field's operations. */
+typedef union { fr_t fr; sf_t sf; } fe_t;
+typedef struct { int field; sf_ctx c; size_t es; int nb; } fctx_t;
+
+static int fctx_make(int field, fctx_t *f) {
+  f->field = field;
+  if (field == FIELD_FR) { f->es = 32; f->nb = 32; return 1; }
+  if (!sf_for(field, &f->c)) return 0;
+  f->es = 8;
+  f->nb = (f->c.bits + 7) / 8;
+  return 1;
+}
+static fe_t fe_load(const fctx_t *f, const void *p) { fe_t x; memset(&x, 0, sizeof x); memcpy(&x, p, f->es); return x; }
+static fe_t fe_add(const fctx_t *f, fe_t a, fe_t b) {
+  if (f->field == FIELD_FR) a.fr = fr_add(a.fr, b.fr); else a.sf = sf_add(&f->c, a.sf, b.sf);
+  return a;
+}
+static fe_t fe_sub(const fctx_t *f, fe_t a, fe_t b) {
+  if (f->field == FIELD_FR) a.fr = fr_sub(a.fr, b.fr); else a.sf = sf_sub(&f->c, a.sf, b.sf);
+  return a;
+}
+static fe_t fe_mul(const fctx_t *f, fe_t a, fe_t b) {
+  if (f->field == FIELD_FR) a.fr = fr_mul(a.fr, b.fr); else a.sf = sf_mul(&f->c, a.sf, b.sf);
+  return a;
+}
+static fe_t fe_inv(const fctx_t *f, fe_t a) {
+  if (f->field == FIELD_FR) a.fr = fr_inv(a.fr); else a.sf = sf_inv(&f->c, a.sf);
+  return a;
+}
+static fe_t fe_from_u64(const fctx_t *f, uint64_t v) {
+  fe_t x;
+  memset(&x, 0, sizeof x);
+  if (f->field == FIELD_FR) x.fr = fr_from_u64(v); else x.sf = sf_from_u64(&f->c, v);
+  return x;
+}
+static fe_t fe_one(const fctx_t *f) { return fe_from_u64(f, 1); }
+static void fe_bytes(const fctx_t *f, fe_t x, uint8_t *out) { to_bytes(f->field, &x, 0, out); }
+
+/* FriError.InvalidParameters */
+#define OR_FRI_INVALID_PARAMETERS 12
+
+/* Config.rounds and Config.validate (root.zig:188-203). */
+static int fri_rounds(size_t log_domain, size_t log_initial_degree, size_t log_final, size_t log_residual_degree,
+                      size_t num_queries, size_t *rounds) {
+  if (log_domain == 0 || log_domain > 63) return OR_FRI_INVALID_PARAMETERS;
+  if (log_final >= log_domain) return OR_FRI_INVALID_PARAMETERS;
+  if (log_residual_degree >= log_final) return OR_FRI_INVALID_PARAMETERS;
+  if (log_final > 12 || log_residual_degree > 12) return OR_FRI_INVALID_PARAMETERS;
+  if (num_queries == 0) return OR_FRI_INVALID_PARAMETERS;
+  const size_t r = log_domain - log_final;
+  if (log_initial_degree > log_domain) return OR_FRI_INVALID_PARAMETERS;
+  if (log_initial_degree < r) return OR_FRI_INVALID_PARAMETERS;
+  if (log_initial_degree - r != log_residual_degree) return OR_FRI_INVALID_PARAMETERS;
+  *rounds = r;
+  return OR_OK;
+}
+
+/* Domain(F).init(F, log_n).step_gen = omega^(2^(two_adicity - log_n)) with
+ * omega = primitiveRootOfUnity(two_adicity) (root.zig:133-138), which is
+ * the primitive root of order 2^log_n of roots.zig:149-157. */
+static fe_t fri_domain_gen(const fctx_t *f, size_t log_n) {
+  fe_t g;
+  memset(&g, 0, sizeof g);
+  oracle_primitive_root(f->field, log_n, &g);
+  return g;
+}
+
+/* Coefficients of the polynomial of degree < m through (g^i, v_i), i < m:
+ * c_j = m^-1 * sum_i v_i g^(-ij), the unique solution of the Vandermonde
+ * system interpolateToCoeffs solves by elimination (root.zig:448-504). */
+static void fri_interpolate(const fctx_t *f, const fe_t *v, size_t m, fe_t g, fe_t *c) {
+  const fe_t ginv = fe_inv(f, g), minv = fe_inv(f, fe_from_u64(f, m));
+  fe_t gj = fe_one(f);  /* g^-j */
+  for (size_t j = 0; j < m; j++) {
+    fe_t acc, x = fe_one(f);  /* g^-ij */
+    memset(&acc, 0, sizeof acc);  /* zero in either representation */
+    for (size_t i = 0; i < m; i++) {
+      acc = fe_add(f, acc, fe_mul(f, v[i], x));
+      x = fe_mul(f, x, gj);
+    }
+    c[j] = fe_mul(f, acc, minv);
+    gj = fe_mul(f, gj, ginv);
+  }
+}
+
+/* prove(F, allocator, transcript, evals, config).  Outputs, in flat arrays:
+ *   roots      rounds x 32          (layers[r].merkle_root; log_size is log_domain - r)
+ *   residual   2^log_residual_degree x NUM_BYTES (serializeElem of each coefficient)
+ *   pair_index num_queries
+ *   values     num_queries x rounds x 2 NUM_BYTES (serializePair)
+ *   paths      num_queries x sum_r (log_domain - r - 1) x 32, round by round,
+ *              each path from the leaf's sibling up (MerkleTree.prove). */
+int oracle_fri_prove(int field, oracle_tr *t, const void *evals, size_t n, size_t log_domain,
+                     size_t log_initial_degree, size_t log_final, size_t log_residual_degree, size_t num_queries,
+                     uint8_t *roots, uint8_t *residual, uint64_t *pair_index, uint8_t *values, uint8_t *paths) {
+  fr_init();
+  fctx_t f;
+  if (!fctx_make(field, &f)) return OR_INVALID_ARGUMENT;
+  size_t rounds;
+  int st = fri_rounds(log_domain, log_initial_degree, log_final, log_residual_degree, num_queries, &rounds);
+  if (st) return st;
+  if (n != ((size_t)1 << log_domain)) return OR_FRI_INVALID_PARAMETERS;
+  const int adicity = field == FIELD_FR ? 28 : f.c.two_adicity;
+  if (log_domain > (size_t)adicity) return OR_FRI_INVALID_PARAMETERS;
+
+  fe_t **layer = (fe_t **)calloc(rounds + 1, sizeof(fe_t *));
+  uint8_t **tree = (uint8_t **)calloc(rounds, sizeof(uint8_t *));
+  layer[0] = (fe_t *)calloc(n, sizeof(fe_t));
+  for (size_t i = 0; i < n; i++) layer[0][i] = fe_load(&f, (const uint8_t *)evals + i * f.es);
+  const fe_t half_inv = fe_inv(&f, fe_from_u64(&f, 2));
+  size_t len = n, log_cur = log_domain;
+  for (size_t r = 0; r < rounds; r++) {
+    const fe_t *cur = layer[r];
+    const size_t half = len / 2;
+    /* commit: leaves over (j, j + half), initFromHashes, root */
+    uint8_t *leaves = (uint8_t *)malloc(32 * half);
+    for (size_t j = 0; j < half; j++) {
+      uint8_t buf[64];
+      fe_bytes(&f, cur[j], buf);
+      fe_bytes(&f, cur[j + half], buf + f.nb);
+      oracle_blake3(buf, (size_t)(2 * f.nb), leaves + 32 * j);
+    }
+    tree[r] = (uint8_t *)malloc(64 * half);
+    oracle_merkle_from_hashes(leaves, half, tree[r], roots + 32 * r);
+    free(leaves);
+    /* absorb the root, squeeze alpha */
+    oracle_tr_absorb(t, roots + 32 * r, 32);
+    fe_t alpha;
+    memset(&alpha, 0, sizeof alpha);
+    oracle_tr_challenge_field(t, field, &alpha);
+    /* fold (root.zig:392-403) */
+    const fe_t g = fri_domain_gen(&f, log_cur);
+    fe_t *next = (fe_t *)calloc(half, sizeof(fe_t));
+    fe_t x = fe_one(&f);
+    for (size_t j = 0; j < half; j++) {
+      const fe_t fx = cur[j], fnx = cur[j + half];
+      const fe_t even = fe_mul(&f, fe_add(&f, fx, fnx), half_inv);
+      const fe_t odd = fe_mul(&f, fe_mul(&f, fe_sub(&f, fx, fnx), fe_inv(&f, x)), half_inv);
+      next[j] = fe_add(&f, even, fe_mul(&f, alpha, odd));
+      x = fe_mul(&f, x, g);
+    }
+    layer[r + 1] = next;
+    len = half;
+    log_cur--;
+  }
+  /* residual: interpolate the final layer, keep 2^log_residual_degree coefficients, absorb them */
+  const size_t m = len, rl = (size_t)1 << log_residual_degree;
+  fe_t *coef = (fe_t *)calloc(m, sizeof(fe_t));
+  fri_interpolate(&f, layer[rounds], m, fri_domain_gen(&f, log_final), coef);
+  for (size_t i = 0; i < rl; i++) {
+    fe_bytes(&f, coef[i], residual + i * (size_t)f.nb);
+    oracle_tr_absorb(t, residual + i * (size_t)f.nb, (size_t)f.nb);
+  }
+  free(coef);
+  /* queries */
+  uint8_t *pp = paths;
+  for (size_t q = 0; q < num_queries; q++) {
+    const uint64_t seed = oracle_tr_challenge_u64(t);
+    size_t e = (size_t)(seed % (uint64_t)n);
+    pair_index[q] = e;
+    size_t lr = n;
+    for (size_t r = 0; r < rounds; r++) {
+      const size_t half = lr / 2, j = e % half;
+      uint8_t *v = values + (q * rounds + r) * 2 * (size_t)f.nb;
+      fe_bytes(&f, layer[r][j], v);
+      fe_bytes(&f, layer[r][j + half], v + f.nb);
+      for (size_t pos = half + j; pos > 1; pos >>= 1) {
+        memcpy(pp, tree[r] + 32 * (pos ^ 1), 32);
+        pp += 32;
+      }
+      e = j;
+      lr = half;
+    }
+  }
+  for (size_t r = 0; r <= rounds; r++) free(layer[r]);
+  for (size_t r = 0; r < rounds; r++) free(tree[r]);
+  free(layer);
+  free(tree);
+  return OR_OK;
+}

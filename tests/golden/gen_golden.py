@@ -12,6 +12,10 @@ Two kinds of fixtures are written:
        libs/ntt/src/root.zig:438-449 (Goldilocks and BabyBear, log_n 1..4);
      * ``ref_field_vectors.json`` -- the add/mul/inv/neg vectors of
        libs/field/src/lib.zig:257-276 (BabyBear and Goldilocks rows);
+     * ``ref_fri_kat.json`` -- the transcript challenge sequence of
+       libs/fri/src/root.zig:1210-1214 (label "wire-compat", four Goldilocks
+       draws, toBytes concatenated) and the Goldilocks FRI domain of
+       :1272 (log_n 8, the 256 elements in natural order);
      * ``ref_blake3_kat.json`` -- the BLAKE3 known answers of
        libs/hash/src/root.zig:121-127 ("hello world") and :173-247 (45
        lengths 0..4097 of a repeating 16-byte pattern).
@@ -113,6 +117,21 @@ def extract_ref_blake3() -> None:
     assert len(pattern) == 16 and len(vecs) == 45, (len(pattern), len(vecs))
     dump("ref_blake3_kat.json", {"source": "libs/hash/src/root.zig:121-127, 173-247",
                                  "hello_world": "".join(hw), "pattern": pattern, "vectors": vecs})
+
+
+def extract_ref_fri() -> None:
+    path = os.path.join(REF, "libs/fri/src/root.zig")
+    text = open(path).read().splitlines()
+    seq = re.search(r'"([0-9a-f]{64})"', text[1211]).group(1)  # root.zig:1212 (Goldilocks)
+    assert "Goldilocks" in text[1209] and "wire-compat" in text[1215], (text[1209], text[1215])
+    line = text[1271]  # root.zig:1272
+    assert "the Goldilocks base-field domain" in line and ".log_n = 8" in line
+    dom = [int(h, 16) for h in re.findall(r'"([0-9a-f]{64})"', line)]
+    assert len(dom) == 256, len(dom)
+    dump("ref_fri_kat.json", {"source": "libs/fri/src/root.zig:1210-1214, 1272",
+                              "transcript": {"label": "wire-compat", "field": "goldilocks", "draws": 4,
+                                             "hex": seq},
+                              "domain": {"field": "goldilocks", "log_n": 8, "values": [hex(v) for v in dom]}})
 
 
 def lcg(seed: int):
@@ -218,11 +237,13 @@ def textbook_digests() -> None:
 
 
 def main() -> None:
-    steps = sys.argv[1:] or ["kat", "field", "blake3", "const", "frvec", "dft", "textbook"]
+    steps = sys.argv[1:] or ["kat", "field", "blake3", "fri", "const", "frvec", "dft", "textbook"]
     if "kat" in steps:
         extract_ref_kat()
     if "blake3" in steps:
         extract_ref_blake3()
+    if "fri" in steps:
+        extract_ref_fri()
     if "field" in steps:
         extract_ref_field_vectors()
     if "const" in steps:

@@ -4,7 +4,10 @@
 * context creation without a GPU fails with ZNTT_NO_DEVICE instead of crashing;
 * host-only entry points (status strings, element sizes, primitive roots)
   agree with the Python-int fixtures;
-* the C oracle's status codes equal the ABI's.
+* the C oracle's status codes equal the ABI's;
+* the host-side BLAKE3 and transcript (no device work) against the
+  reference's known answers and the oracle's transcript, and the FRI proof
+  sizes against the oracle's.
 """
 import json
 import os
@@ -47,8 +50,9 @@ def test_status_codes_match_oracle_and_reference_error_names(Z):
     assert Z.OutOfMemory.status == O.OUT_OF_MEMORY
     assert Z.OrderTooLarge.status == O.ORDER_TOO_LARGE
     assert Z.DegreeTooLarge.status == O.DEGREE_TOO_LARGE
+    assert Z.InvalidParameters.status == O.FRI_INVALID_PARAMETERS
     for cls in (Z.LogTooLarge, Z.LengthMismatch, Z.InvalidLength, Z.InvalidTwiddles, Z.OutOfMemory,
-                Z.OrderTooLarge, Z.DegreeTooLarge):
+                Z.OrderTooLarge, Z.DegreeTooLarge, Z.InvalidParameters):
         assert Z.lib.zntt_status_string(cls.status).decode() == cls.__name__
 
 
@@ -120,3 +124,43 @@ def test_workload_metadata_generators():
     assert 1800 < len(arr) < 2200
     assert all(12 <= k <= 22 and b in (2, 3) for _, k, b in arr)
     assert all(t0 < t1 for (t0, _, _), (t1, _, _) in zip(arr, arr[1:]))
+
+
+def test_host_blake3_reference_known_answers(Z):
+    k = json.load(open(os.path.join(ROOT, "tests", "golden", "ref_blake3_kat.json")))
+    pat = bytes(k["pattern"])
+    data = (pat * (4097 // len(pat) + 1))[:4097]
+    assert Z.blake3(b"hello world").hex() == k["hello_world"]
+    for v in k["vectors"]:
+        assert Z.blake3(data[: v["len"]]).hex() == v["digest"], v["len"]
+
+
+def test_transcript_reference_sequence_and_oracle(Z):
+    k = json.load(open(os.path.join(ROOT, "tests", "golden", "ref_fri_kat.json")))["transcript"]
+    t = Z.Transcript(k["label"].encode())
+    seq = b""
+    for _ in range(k["draws"]):  # challengeFieldChecked over Goldilocks
+        while True:
+            b = t.challenge_bytes(8)
+            if int.from_bytes(b, "little") < O.GL_MODULUS:
+                break
+        seq += b
+    assert seq.hex() == k["hex"]
+    # A mixed script of absorbs and squeezes of every length class.
+    a, b = Z.Transcript(b"script"), O.Transcript(b"script")
+    for i, (absorb, squeeze) in enumerate([(0, 8), (3, 32), (1100, 40), (64, 1), (5000, 100), (0, 0)]):
+        data = bytes((i * 37 + j) % 251 for j in range(absorb))
+        a.absorb_bytes(data)
+        b.absorb_bytes(data)
+        assert a.challenge_bytes(squeeze) == b.challenge_bytes(squeeze), i
+
+
+def test_fri_proof_sizes_match_oracle(Z):
+    for field in (Z.FIELD_FR, Z.FIELD_GL, Z.FIELD_BB, Z.FIELD_KB):
+        for cfg in [(8, 5, 6, 3, 8), (20, 17, 8, 5, 80), (9, 5, 4, 0, 3)]:
+            ld, li, lf, lr, q = cfg
+            got = Z.fri_proof_sizes(field, ld, li, lf, lr, q)
+            want = O.fri_proof_sizes(field, ld, lf, lr, q)
+            assert got == {k: want[k] for k in got}, (field, cfg)
+    with pytest.raises(Z.InvalidParameters):
+        Z.fri_proof_sizes(Z.FIELD_GL, 8, 4, 6, 3, 8)

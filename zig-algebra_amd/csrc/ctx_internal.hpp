@@ -90,5 +90,10 @@ int dev_alloc(zntt_ctx *c, size_t bytes, void **out);
 int ensure(zntt_ctx *c, void **buf, size_t *have, size_t bytes);
 int power_table(zntt_ctx *c, int field, const Elem &root, int L, hipStream_t s, TwiddleEntry **out);
 int coset_table(zntt_ctx *c, int field, const Elem &shift, size_t count, hipStream_t s, const uint32_t **out);
+// One transform of 2^L user-layout values on the device (tw: table of the
+// root's powers, coset: optional shift powers, scale: optional dev-form
+// multiplier of every output).
+int run_device(zntt_ctx *c, int field, int L, void *d_data, const uint32_t *tw, const uint32_t *coset,
+               const std::array<uint32_t, 9> *scale, hipStream_t s, bool tw_one = true);
 
 }  // namespace zntt_detail

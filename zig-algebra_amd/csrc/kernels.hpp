@@ -89,6 +89,13 @@ hipError_t launch_merkle_layer(int field, const void *evals, int leaf_log, uint3
 // Tree over `count` stored hashes padded to 2^cap_log leaves.
 hipError_t launch_merkle_from_hashes(const void *hashes, uint64_t count, int cap_log, uint32_t *heap, bool full,
                                      hipStream_t s);
+// One FRI fold (fri_kernels.hip): next[j] = (cur[j] + cur[j+half]) / 2 +
+// alpha (cur[j] - cur[j+half]) / (2 g^j), user layout, with inv_pows[j] =
+// g^-j (dev layout) and the dev-form constants alpha/2 and 1/2.
+hipError_t launch_fri_fold(int field, const void *cur, void *next, uint64_t half, const uint32_t *inv_pows,
+                           const uint32_t alpha_half_dev[9], const uint32_t half_dev[9], hipStream_t s);
+// out[i] = `words` u32 words read at device address addrs[i] (addrs on the device).
+hipError_t launch_gather(const uint64_t *addrs, uint64_t count, int words, uint32_t *out, hipStream_t s);
 // Elementwise device field op (0 add, 1 sub, 2 mul) on user-layout arrays.
 hipError_t launch_field_op(int field, int op, size_t n, const uint32_t *a, const uint32_t *b, uint32_t *out,
                            hipStream_t stream);

@@ -300,7 +300,7 @@ int check_transform_args(zntt_ctx *c, int field, const void *data, size_t len, s
 // Runs one transform over device data on stream s.  `tw` covers >= 2^L - 1
 // entries; `coset` (nullable) covers 2^L; `scale` (nullable) is a dev constant.
 int run_device(zntt_ctx *c, int field, int L, void *d_data, const uint32_t *tw, const uint32_t *coset,
-               const std::array<uint32_t, 9> *scale, hipStream_t s, bool tw_one = true) {
+               const std::array<uint32_t, 9> *scale, hipStream_t s, bool tw_one) {
   if (L == 0) {
     if (!scale && !coset) return ZNTT_OK;
   }
@@ -498,6 +498,7 @@ const char *zntt_status_string(int status) {
     case ZNTT_UNSUPPORTED: return "Unsupported";
     case ZNTT_ORDER_TOO_LARGE: return "OrderTooLarge";
     case ZNTT_DEGREE_TOO_LARGE: return "DegreeTooLarge";
+    case ZNTT_INVALID_PARAMETERS: return "InvalidParameters";
     default: return "unknown";
   }
 }

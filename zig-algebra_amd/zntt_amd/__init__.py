@@ -81,9 +81,13 @@ class DegreeTooLarge(ZnttError):
     status = 11
 
 
+class InvalidParameters(ZnttError):
+    status = 12
+
+
 _ERRORS = {c.status: c for c in (LogTooLarge, LengthMismatch, InvalidLength, InvalidTwiddles, OutOfMemory,
                                  DeviceError, NoDevice, InvalidArgument, Unsupported, OrderTooLarge,
-                                 DegreeTooLarge)}
+                                 DegreeTooLarge, InvalidParameters)}
 
 # Every symbol include/zntt_gpu.h declares.
 EXPORTED = [
@@ -98,7 +102,26 @@ EXPORTED = [
     "zntt_server_read", "zntt_server_release", "zntt_server_drain", "zntt_server_stats_get", "zntt_server_sim_pump",
     "zntt_server_sim_complete", "zntt_server_destroy", "zntt_fill_counter",
     "zntt_poly_mul", "zntt_poly_mul_device", "zntt_merkle_commit", "zntt_merkle_from_hashes",
+    "zntt_fri_proof_sizes", "zntt_fri_prove", "zntt_transcript_create", "zntt_transcript_destroy",
+    "zntt_transcript_absorb_bytes", "zntt_transcript_challenge_bytes", "zntt_blake3",
 ]
+
+BYTES_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint8), ctypes.c_size_t)
+
+
+class TranscriptOps(ctypes.Structure):
+    _fields_ = [("absorb_bytes", BYTES_FN), ("challenge_bytes", BYTES_FN)]
+
+
+class FriConfig(ctypes.Structure):
+    _fields_ = [("log_domain", ctypes.c_size_t), ("log_initial_degree", ctypes.c_size_t),
+                ("log_final", ctypes.c_size_t), ("log_residual_degree", ctypes.c_size_t),
+                ("num_queries", ctypes.c_size_t)]
+
+
+class FriProof(ctypes.Structure):
+    _fields_ = [("roots", ctypes.c_void_p), ("residual", ctypes.c_void_p), ("pair_index", ctypes.c_void_p),
+                ("values", ctypes.c_void_p), ("paths", ctypes.c_void_p)]
 
 REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED, REQ_CANCELLED = 0, 1, 2, 3, 4
 
@@ -206,6 +229,18 @@ def _load():
     L.zntt_poly_mul_device.argtypes = [vp, i32, vp, sz, vp, sz, vp, vp]
     L.zntt_merkle_commit.argtypes = [vp, i32, vp, sz, vp, vp, vp]
     L.zntt_merkle_from_hashes.argtypes = [vp, vp, sz, vp, vp, vp]
+    L.zntt_fri_proof_sizes.argtypes = [i32, ctypes.POINTER(FriConfig)] + [ctypes.POINTER(sz)] * 4
+    L.zntt_fri_prove.argtypes = [vp, i32, ctypes.POINTER(FriConfig), ctypes.POINTER(TranscriptOps), vp, vp, sz,
+                                 ctypes.POINTER(FriProof), vp]
+    L.zntt_transcript_create.argtypes = [ctypes.c_char_p, sz, ctypes.POINTER(vp)]
+    L.zntt_transcript_destroy.argtypes = [vp]
+    L.zntt_transcript_destroy.restype = None
+    L.zntt_transcript_absorb_bytes.argtypes = [vp, ctypes.c_char_p, sz]
+    L.zntt_transcript_absorb_bytes.restype = None
+    L.zntt_transcript_challenge_bytes.argtypes = [vp, vp, sz]
+    L.zntt_transcript_challenge_bytes.restype = None
+    L.zntt_blake3.argtypes = [ctypes.c_char_p, sz, vp]
+    L.zntt_blake3.restype = None
     return L
 
 
@@ -264,6 +299,68 @@ def primitive_root(field: int, log_size: int) -> np.ndarray:
     out = empty(field, 1).reshape(-1)
     check(lib.zntt_primitive_root(field, log_size, _ptr(out)))
     return out
+
+
+def blake3(data: bytes) -> bytes:
+    """BLAKE3 of `data` (32 bytes), the library's host implementation."""
+    out = (ctypes.c_uint8 * 32)()
+    lib.zntt_blake3(bytes(data), len(data), out)
+    return bytes(out)
+
+
+class Transcript:
+    """The library's Fiat-Shamir transcript (zig-transcript's byte stream)."""
+
+    def __init__(self, label: bytes):
+        h = ctypes.c_void_p()
+        check(lib.zntt_transcript_create(bytes(label), len(label), ctypes.byref(h)))
+        self._h = h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) and lib is not None:
+            lib.zntt_transcript_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def absorb_bytes(self, data: bytes) -> None:
+        lib.zntt_transcript_absorb_bytes(self._h, bytes(data), len(data))
+
+    def challenge_bytes(self, n: int) -> bytes:
+        out = (ctypes.c_uint8 * max(n, 1))()
+        lib.zntt_transcript_challenge_bytes(self._h, out, n)
+        return bytes(out)[:n]
+
+    def ops(self):
+        """(ops, handle) for zntt_fri_prove: the library's own functions."""
+        return (TranscriptOps(ctypes.cast(lib.zntt_transcript_absorb_bytes, BYTES_FN),
+                              ctypes.cast(lib.zntt_transcript_challenge_bytes, BYTES_FN)), self._h)
+
+
+def callback_ops(obj):
+    """(ops, handle, keepalive) that route zntt_fri_prove's transcript calls to
+    any Python object with absorb_bytes(bytes) and challenge_bytes(n) -> bytes."""
+    def absorb(_, data, n):
+        obj.absorb_bytes(ctypes.string_at(data, n))
+
+    def challenge(_, out, n):
+        b = obj.challenge_bytes(n)
+        ctypes.memmove(out, b, n)
+
+    fa, fc = BYTES_FN(absorb), BYTES_FN(challenge)
+    return TranscriptOps(fa, fc), None, (fa, fc)
+
+
+def fri_proof_sizes(field: int, log_domain: int, log_initial_degree: int, log_final: int,
+                    log_residual_degree: int, num_queries: int) -> dict:
+    cfg = FriConfig(log_domain, log_initial_degree, log_final, log_residual_degree, num_queries)
+    v = [ctypes.c_size_t() for _ in range(4)]
+    check(lib.zntt_fri_proof_sizes(field, ctypes.byref(cfg), *[ctypes.byref(x) for x in v]))
+    return dict(zip(["roots", "residual", "values", "paths"], [x.value for x in v]))
 
 
 class Context:
@@ -373,6 +470,32 @@ class Context:
         root = (ctypes.c_uint8 * 32)()
         check(lib.zntt_merkle_from_hashes(self._h, hashes_ptr, count, nodes_ptr, root, stream))
         return bytes(root)
+
+    def fri_prove(self, field: int, transcript, evals_ptr: int, evals_len: int, log_domain: int,
+                  log_initial_degree: int, log_final: int, log_residual_degree: int, num_queries: int,
+                  stream: int | None = None) -> dict:
+        """FRI prove over device evaluations; `transcript` is a Transcript or any
+        object with absorb_bytes / challenge_bytes (called back)."""
+        if stream is None:
+            stream = lib.zntt_ctx_stream(self._h)
+        cfg = FriConfig(log_domain, log_initial_degree, log_final, log_residual_degree, num_queries)
+        sz = fri_proof_sizes(field, log_domain, log_initial_degree, log_final, log_residual_degree, num_queries)
+        bufs = {k: (ctypes.c_uint8 * max(v, 1))() for k, v in sz.items()}
+        pair = (ctypes.c_uint64 * num_queries)()
+        proof = FriProof(ctypes.cast(bufs["roots"], ctypes.c_void_p), ctypes.cast(bufs["residual"], ctypes.c_void_p),
+                         ctypes.cast(pair, ctypes.c_void_p), ctypes.cast(bufs["values"], ctypes.c_void_p),
+                         ctypes.cast(bufs["paths"], ctypes.c_void_p))
+        if isinstance(transcript, Transcript):
+            ops, handle = transcript.ops()
+            keep = None
+        else:
+            ops, handle, keep = callback_ops(transcript)
+        check(lib.zntt_fri_prove(self._h, field, ctypes.byref(cfg), ctypes.byref(ops), handle, evals_ptr,
+                                 evals_len, ctypes.byref(proof), stream))
+        del keep
+        out = {k: bytes(bufs[k])[:sz[k]] for k in sz}
+        out["pair_index"] = [int(x) for x in pair]
+        return out
 
     def lde(self, field: int, coeffs: np.ndarray, log_k: int, log_blowup: int, shift) -> np.ndarray:
         out = empty(field, 1 << (log_k + log_blowup))
