@@ -445,7 +445,8 @@ int zntt_fri_proof_sizes(int field, const zntt_fri_config *cfg, size_t *roots_by
  * log_domain above the field's two-adicity; ZNTT_UNSUPPORTED beyond
  * ZNTT_MAX_LOG_N.  Work runs on `stream`; the call returns when the proof is
  * written.  Device memory: about 2^log_domain x (value + 32) bytes for the
- * folded layers and the trees, freed before returning. */
+ * folded layers and the trees, context scratch kept for the next call
+ * (zntt_ctx_trim releases it). */
 int zntt_fri_prove(zntt_ctx *ctx, int field, const zntt_fri_config *cfg, const zntt_transcript_ops *ops,
                    void *transcript, const void *evals, size_t evals_len, zntt_fri_proof *proof, void *stream);
 

@@ -61,6 +61,8 @@ struct zntt_ctx {
   size_t staging_bytes = 0;
   void *aux = nullptr;  // foreign tables and other per-call temporaries
   size_t aux_bytes = 0;
+  void *pinned = nullptr;  // page-locked host staging (small read-backs and uploads)
+  size_t pinned_bytes = 0;
 };
 
 namespace zntt_detail {
@@ -88,6 +90,8 @@ int hip_status(hipError_t e);
 void dev_free(zntt_ctx *c, void *p, size_t bytes);
 int dev_alloc(zntt_ctx *c, size_t bytes, void **out);
 int ensure(zntt_ctx *c, void **buf, size_t *have, size_t bytes);
+// Grows the context's page-locked host buffer to at least `bytes`.
+int ensure_pinned(zntt_ctx *c, size_t bytes);
 int power_table(zntt_ctx *c, int field, const Elem &root, int L, hipStream_t s, TwiddleEntry **out);
 int coset_table(zntt_ctx *c, int field, const Elem &shift, size_t count, hipStream_t s, const uint32_t **out);
 // One transform of 2^L user-layout values on the device (tw: table of the

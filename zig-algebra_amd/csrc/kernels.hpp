@@ -94,8 +94,16 @@ hipError_t launch_merkle_from_hashes(const void *hashes, uint64_t count, int cap
 // g^-j (dev layout) and the dev-form constants alpha/2 and 1/2.
 hipError_t launch_fri_fold(int field, const void *cur, void *next, uint64_t half, const uint32_t *inv_pows,
                            const uint32_t alpha_half_dev[9], const uint32_t half_dev[9], hipStream_t s);
-// out[i] = `words` u32 words read at device address addrs[i] (addrs on the device).
-hipError_t launch_gather(const uint64_t *addrs, uint64_t count, int words, uint32_t *out, hipStream_t s);
+// FRI query openings: for each of q queries (pair index e0[k]) and each of
+// the `rounds` layers, the antipodal pair (2 elements, user layout) into
+// vals and the Merkle path (32-byte nodes) into paths.
+struct FriOpenArgs {
+  const void *layer[64];  // layer r (user layout), r < rounds
+  const void *tree[64];   // layer r's node array
+  uint32_t rounds, log_domain, elem_words;
+};
+hipError_t launch_fri_open(const FriOpenArgs &a, const uint64_t *e0, uint64_t q, uint32_t *vals, uint32_t *paths,
+                           hipStream_t s);
 // Elementwise device field op (0 add, 1 sub, 2 mul) on user-layout arrays.
 hipError_t launch_field_op(int field, int op, size_t n, const uint32_t *a, const uint32_t *b, uint32_t *out,
                            hipStream_t stream);

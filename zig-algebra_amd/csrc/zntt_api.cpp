@@ -172,6 +172,25 @@ int ensure(zntt_ctx *c, void **buf, size_t *have, size_t bytes) {
   return st;
 }
 
+int ensure_pinned(zntt_ctx *c, size_t bytes) {
+  if (c->pinned_bytes >= bytes) return ZNTT_OK;
+  if (c->pinned) {
+    hipStreamSynchronize(c->stream);
+    hipDeviceSynchronize();
+    hipHostFree(c->pinned);
+    c->pinned = nullptr;
+    c->pinned_bytes = 0;
+  }
+  hipError_t e = hipHostMalloc(&c->pinned, bytes, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    hipGetLastError();
+    c->pinned = nullptr;
+    return hip_status(e);
+  }
+  c->pinned_bytes = bytes;
+  return ZNTT_OK;
+}
+
 std::vector<Elem> root_chain(int field, const Elem &root, int L) {
   std::vector<Elem> ch(L + 1);
   ch[0] = root;
@@ -465,6 +484,9 @@ int zntt_ctx_trim(zntt_ctx *c) {
   dev_free(c, c->aux, c->aux_bytes);
   c->scratch = c->staging = c->aux = nullptr;
   c->scratch_bytes = c->staging_bytes = c->aux_bytes = 0;
+  if (c->pinned) hipHostFree(c->pinned);
+  c->pinned = nullptr;
+  c->pinned_bytes = 0;
   return ZNTT_OK;
 }
 

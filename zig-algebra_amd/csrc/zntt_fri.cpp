@@ -3,11 +3,13 @@
 // (libs/transcript/src/root.zig:38-121).
 //
 // Device: each round's Merkle commitment (merkle_kernels.hip), each fold
-// (fri_kernels.hip), the residual's inverse transform and the gathers of
-// the opened values and paths.  Host: the transcript (through the caller's
+// (fri_kernels.hip), the residual's inverse transform and the query
+// openings (values and paths, one launch).  Host: the transcript (through the caller's
 // callbacks), the challenge decoding, and the byte layout of the proof.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <new>
 #include <vector>
 
 #include "ctx_internal.hpp"
@@ -110,21 +112,6 @@ size_t path_entries(const zntt_fri_config *c, size_t rounds) {
   return d;
 }
 
-// Device buffers of one prove call, released on every exit.
-struct Buffers {
-  zntt_ctx *c;
-  std::vector<std::pair<void *, size_t>> held;
-  explicit Buffers(zntt_ctx *ctx) : c(ctx) {}
-  ~Buffers() {
-    for (auto &h : held) dev_free(c, h.first, h.second);
-  }
-  int get(size_t bytes, void **out) {
-    int st = dev_alloc(c, bytes, out);
-    if (st == ZNTT_OK) held.emplace_back(*out, bytes);
-    return st;
-  }
-};
-
 bool aligned16(const void *p) { return ((uintptr_t)p & 15) == 0; }
 
 int prove(zntt_ctx *c, int field, const zntt_fri_config *cfg, const zntt_transcript_ops *ops, void *tr,
@@ -138,39 +125,54 @@ int prove(zntt_ctx *c, int field, const zntt_fri_config *cfg, const zntt_transcr
   if (ld > ZNTT_MAX_LOG_N) return ZNTT_UNSUPPORTED;
   const size_t n = evals_len, ub = user_bytes(field), nb = num_bytes(field);
 
-  // Layers 1..R (layer 0 is the caller's buffer) and the R trees, whose
-  // node arrays hold n_r x 32 bytes each.
-  Buffers buf(c);
+  // Context memory (kept between calls, grown when a larger proof needs
+  // it): layers 1..R (layer 0 is the caller's buffer), the R trees, whose
+  // node arrays hold n_r x 32 bytes each, the query seeds and the opened
+  // values and paths.  Page-locked host memory stages the roots, the
+  // residual, the seeds and the openings.
+  const size_t q = cfg->num_queries, depth = path_entries(cfg, rounds);
+  const size_t nvals = q * rounds * 2, npaths = q * depth;
+  size_t lbytes = 0, tbytes = 0;
+  for (size_t r = 0; r < rounds; r++) {
+    lbytes += (n >> (r + 1)) * ub;
+    tbytes += (n >> r) * 32;
+  }
+  const size_t rl = (size_t)1 << cfg->log_residual_degree;
+  const size_t ebytes = (q * 8 + 15) & ~(size_t)15, obytes = nvals * ub + npaths * 32;
+  if ((st = ensure(c, &c->aux, &c->aux_bytes, lbytes + tbytes + ebytes + obytes))) return st;
+  const size_t hbytes = ebytes + std::max(obytes, std::max(rl * ub, (size_t)32));
+  if ((st = ensure_pinned(c, hbytes))) return st;
+  uint8_t *pin = (uint8_t *)c->pinned, *pin_out = pin + ebytes;
   std::vector<uint8_t *> layer(rounds + 1), tree(rounds);
   layer[0] = (uint8_t *)evals;
-  {
-    size_t lbytes = 0, tbytes = 0;
-    for (size_t r = 0; r < rounds; r++) {
-      lbytes += (n >> (r + 1)) * ub;
-      tbytes += (n >> r) * 32;
-    }
-    void *lp = nullptr, *tp = nullptr;
-    if ((st = buf.get(lbytes, &lp)) || (st = buf.get(tbytes, &tp))) return st;
-    uint8_t *l = (uint8_t *)lp, *t = (uint8_t *)tp;
-    for (size_t r = 0; r < rounds; r++) {
-      layer[r + 1] = l;
-      l += (n >> (r + 1)) * ub;
-      tree[r] = t;
-      t += (n >> r) * 32;
-    }
+  uint8_t *mem = (uint8_t *)c->aux;
+  for (size_t r = 0; r < rounds; r++) {
+    layer[r + 1] = mem;
+    mem += (n >> (r + 1)) * ub;
   }
+  for (size_t r = 0; r < rounds; r++) {
+    tree[r] = mem;
+    mem += (n >> r) * 32;
+  }
+  uint64_t *d_seeds = (uint64_t *)mem;
+  uint8_t *d_out = mem + ebytes;
 
   const Elem half_inv = f_inv(field, f_from_u64(field, 2));
   const std::array<uint32_t, 9> half_dev = f_to_dev(field, half_inv);
+  // Domain generators: g_r (order 2^(ld - r)) is the primitive root of that
+  // order, and g_{r+1} = g_r^2 (Domain.init, root.zig:133-138), so one root
+  // and one inversion serve every round.
+  Elem ginv = f_inv(field, f_primitive_root(field, (int)ld));
   for (size_t r = 0; r < rounds; r++) {
     const int log_cur = (int)(ld - r);
     const size_t half = n >> (r + 1);
     // Commit (root.zig:378-384).
     hipError_t e = zntt::launch_merkle_layer(field, layer[r], log_cur - 1, (uint32_t *)tree[r], true, s);
     uint8_t *root = proof->roots + 32 * r;
-    if (e == hipSuccess) e = hipMemcpyAsync(root, tree[r] + 32, 32, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(pin_out, tree[r] + 32, 32, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return hip_status(e);
+    memcpy(root, pin_out, 32);
     // Absorb the root, squeeze alpha (:387-389).
     ops->absorb_bytes(tr, root, 32);
     const Elem alpha = challenge_field(ops, tr, field);
@@ -178,8 +180,8 @@ int prove(zntt_ctx *c, int field, const zntt_fri_config *cfg, const zntt_transcr
     // the layer's domain generator (Domain.init: omega^(2^(adicity - k)),
     // the primitive root of order 2^k).
     const uint32_t *tab = nullptr;
-    const Elem ginv = f_inv(field, f_primitive_root(field, log_cur));
     if ((st = coset_table(c, field, ginv, half, s, &tab))) return st;
+    ginv = f_mul(field, ginv, ginv);
     const std::array<uint32_t, 9> ah_dev = f_to_dev(field, f_mul(field, alpha, half_inv));
     e = zntt::launch_fri_fold(field, layer[r], layer[r + 1], half, tab, ah_dev.data(), half_dev.data(), s);
     if (e != hipSuccess) return hip_status(e);
@@ -190,61 +192,45 @@ int prove(zntt_ctx *c, int field, const zntt_fri_config *cfg, const zntt_transcr
   // system interpolateToCoeffs eliminates -- then the first
   // 2^log_residual_degree of them, serialised and absorbed in order.
   const int lf = (int)cfg->log_final;
-  const size_t m = (size_t)1 << lf, rl = (size_t)1 << cfg->log_residual_degree;
+  const size_t m = (size_t)1 << lf;
   {
     TwiddleEntry *inv = nullptr;
-    const Elem gf = f_primitive_root(field, lf);
-    if ((st = power_table(c, field, f_inv(field, gf), lf, s, &inv))) return st;
+    if ((st = power_table(c, field, ginv, lf, s, &inv))) return st;  // g_R^-1, R = ld - lf
     const std::array<uint32_t, 9> scale = f_to_dev(field, f_inv(field, f_from_u64(field, m)));
     if ((st = run_device(c, field, lf, layer[rounds], inv->table, nullptr, &scale, s))) return st;
-    std::vector<uint8_t> coef(rl * ub);
-    hipError_t e = hipMemcpyAsync(coef.data(), layer[rounds], rl * ub, hipMemcpyDeviceToHost, s);
+    hipError_t e = hipMemcpyAsync(pin_out, layer[rounds], rl * ub, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return hip_status(e);
     for (size_t i = 0; i < rl; i++) {
-      to_bytes(field, coef.data() + i * ub, proof->residual + i * nb);
+      to_bytes(field, pin_out + i * ub, proof->residual + i * nb);
       ops->absorb_bytes(tr, proof->residual + i * nb, nb);
     }
   }
 
-  // Queries (buildQueries, :546-570): draw every index first, then gather the
-  // opened pairs and the Merkle paths from the device in two launches.
-  const size_t q = cfg->num_queries, depth = path_entries(cfg, rounds);
-  std::vector<uint64_t> addr;
-  addr.reserve(q * rounds * 2 + q * depth);
-  std::vector<size_t> js(q * rounds);
+  // Queries (buildQueries, :546-570): draw every pair index, then open all
+  // of them on the device in one launch (the slot of round r is the index
+  // modulo the round's half size).
+  uint64_t *seeds = (uint64_t *)pin;
   for (size_t k = 0; k < q; k++) {
     const uint64_t seed = challenge_u64(ops, tr);
-    size_t idx = (size_t)(seed % (uint64_t)n);
-    proof->pair_index[k] = idx;
-    for (size_t r = 0; r < rounds; r++) {
-      const size_t half = n >> (r + 1), j = idx % half;
-      js[k * rounds + r] = j;
-      addr.push_back((uint64_t)(uintptr_t)(layer[r] + j * ub));
-      addr.push_back((uint64_t)(uintptr_t)(layer[r] + (j + half) * ub));
-      idx = j;
-    }
+    proof->pair_index[k] = seeds[k] = seed % (uint64_t)n;
   }
-  const size_t nvals = addr.size();
-  for (size_t k = 0; k < q; k++)
-    for (size_t r = 0; r < rounds; r++)
-      for (size_t pos = (n >> (r + 1)) + js[k * rounds + r]; pos > 1; pos >>= 1)
-        addr.push_back((uint64_t)(uintptr_t)(tree[r] + 32 * (pos ^ 1)));
-  const size_t npaths = addr.size() - nvals;
-  void *d_addr = nullptr, *d_out = nullptr;
-  const size_t out_bytes = nvals * ub + npaths * 32;
-  if ((st = buf.get(addr.size() * 8, &d_addr)) || (st = buf.get(out_bytes, &d_out))) return st;
-  std::vector<uint8_t> host(out_bytes);
-  hipError_t e = hipMemcpyAsync(d_addr, addr.data(), addr.size() * 8, hipMemcpyHostToDevice, s);
-  if (e == hipSuccess) e = zntt::launch_gather((const uint64_t *)d_addr, nvals, (int)(ub / 4), (uint32_t *)d_out, s);
+  zntt::FriOpenArgs oa = {};
+  for (size_t r = 0; r < rounds; r++) {
+    oa.layer[r] = layer[r];
+    oa.tree[r] = tree[r];
+  }
+  oa.rounds = (uint32_t)rounds;
+  oa.log_domain = (uint32_t)ld;
+  oa.elem_words = (uint32_t)(ub / 4);
+  hipError_t e = hipMemcpyAsync(d_seeds, seeds, q * 8, hipMemcpyHostToDevice, s);
   if (e == hipSuccess)
-    e = zntt::launch_gather((const uint64_t *)d_addr + nvals, npaths, 8, (uint32_t *)((uint8_t *)d_out + nvals * ub),
-                            s);
-  if (e == hipSuccess) e = hipMemcpyAsync(host.data(), d_out, out_bytes, hipMemcpyDeviceToHost, s);
+    e = zntt::launch_fri_open(oa, d_seeds, q, (uint32_t *)d_out, (uint32_t *)(d_out + nvals * ub), s);
+  if (e == hipSuccess) e = hipMemcpyAsync(pin_out, d_out, obytes, hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   if (e != hipSuccess) return hip_status(e);
-  for (size_t i = 0; i < nvals; i++) to_bytes(field, host.data() + i * ub, proof->values + i * nb);
-  if (npaths) memcpy(proof->paths, host.data() + nvals * ub, npaths * 32);
+  for (size_t i = 0; i < nvals; i++) to_bytes(field, pin_out + i * ub, proof->values + i * nb);
+  if (npaths) memcpy(proof->paths, pin_out + nvals * ub, npaths * 32);
   return ZNTT_OK;
 }
 
