@@ -86,14 +86,20 @@ int merkle_heap_log(int cap_log, bool full);
 // and j + 2^leaf_log.
 hipError_t launch_merkle_layer(int field, const void *evals, int leaf_log, uint32_t *heap, bool full,
                                hipStream_t s);
+// Fold the previous FRI layer (2^(leaf_log + 2) values, user layout) into
+// `next` (2^(leaf_log + 1) values) and commit `next` in the same launch: the
+// whole tree into heap, as launch_merkle_layer with `full`.  inv_pows[x] =
+// g^-x for the previous layer's generator g, as launch_fri_fold.
+hipError_t launch_merkle_fold(int field, const void *prev, void *next, int leaf_log, uint32_t *heap,
+                              const uint32_t *inv_pows, const uint32_t alpha_dev[9], hipStream_t s);
 // Tree over `count` stored hashes padded to 2^cap_log leaves.
 hipError_t launch_merkle_from_hashes(const void *hashes, uint64_t count, int cap_log, uint32_t *heap, bool full,
                                      hipStream_t s);
 // One FRI fold (fri_kernels.hip): next[j] = (cur[j] + cur[j+half]) / 2 +
 // alpha (cur[j] - cur[j+half]) / (2 g^j), user layout, with inv_pows[j] =
-// g^-j (dev layout) and the dev-form constants alpha/2 and 1/2.
+// g^-j (dev layout) and alpha in dev form.
 hipError_t launch_fri_fold(int field, const void *cur, void *next, uint64_t half, const uint32_t *inv_pows,
-                           const uint32_t alpha_half_dev[9], const uint32_t half_dev[9], hipStream_t s);
+                           const uint32_t alpha_dev[9], hipStream_t s);
 // FRI query openings: for each of q queries (pair index e0[k]) and each of
 // the `rounds` layers, the antipodal pair (2 elements, user layout) into
 // vals and the Merkle path (32-byte nodes) into paths.

@@ -34,6 +34,7 @@
 #include <stdint.h>
 
 #include "field_traits.hpp"
+#include "fri_fold.hpp"
 #include "kernels.hpp"
 
 namespace zntt {
@@ -195,6 +196,54 @@ struct LayerLeaves31 {
   }
 };
 
+// Leaf words of a fold output (canonical): its toBytes words (BN254 Fr: out
+// of the Montgomery form).
+__device__ __forceinline__ int leaf_words(const FrT::E &v, uint32_t *w) {
+  fr_canonical_words(v, w);
+  return 8;
+}
+__device__ __forceinline__ int leaf_words(uint64_t v, uint32_t *w) {
+  w[0] = (uint32_t)v;
+  w[1] = (uint32_t)(v >> 32);
+  return 2;
+}
+__device__ __forceinline__ int leaf_words(uint32_t v, uint32_t *w) {
+  w[0] = v;
+  return 1;
+}
+
+// Children of a launch that folds the previous FRI layer and commits the
+// new one: leaf j of the new layer (2 * half values) hashes new values j and
+// j + half, each folded here from the previous layer's pair (x, x + 2 half)
+// and stored to the new layer on the way.  Every new value is folded once:
+// get2(j) covers j, j + 1, j + half and j + 1 + half.
+template <class F>
+struct FoldLeaves {
+  using E = typename F::E;
+  const uint32_t *prev;
+  uint32_t *next;
+  const uint32_t *inv_pows;  // g^-x, dev layout
+  uint64_t half;             // leaf count of the new layer
+  FoldConsts k;
+  __device__ E fold_at(uint64_t x) const {
+    const E a = F::load_user(prev, x), b = F::load_user(prev, x + 2 * half);
+    const E v = FoldOps<F>::fold(a, b, F::load_dev(inv_pows, x), F::from_words(k.alpha));
+    FoldOps<F>::store(next, x, v);
+    return v;
+  }
+  __device__ static void leaf(const E &u, const E &v, uint32_t h[8]) {
+    uint32_t m[16] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    const int w = leaf_words(u, m);
+    leaf_words(v, m + w);
+    hash_block(m, 8 * w, h);
+  }
+  __device__ void get(uint64_t j, uint32_t h[8]) const { leaf(fold_at(j), fold_at(j + half), h); }
+  __device__ void get2(uint64_t j, uint32_t (&out)[16]) const {
+    get(j, out);
+    get(j + 1, out + 8);
+  }
+};
+
 // Children that are already hashes (count of them, then BLAKE3("") padding).
 struct StoredNodes {
   const uint32_t *nodes;
@@ -323,6 +372,19 @@ hipError_t launch_merkle_layer(int field, const void *evals, int leaf_log, uint3
   if (field == kFr) return build_tree(LayerLeaves<FrT>{e, half}, leaf_log, heap, full, s);
   if (field == kGl) return build_tree(LayerLeaves<GlT>{e, half}, leaf_log, heap, full, s);
   return build_tree(LayerLeaves31{e, half}, leaf_log, heap, full, s);
+}
+
+hipError_t launch_merkle_fold(int field, const void *prev, void *next, int leaf_log, uint32_t *heap,
+                              const uint32_t *inv_pows, const uint32_t alpha_dev[9], hipStream_t s) {
+  FoldConsts k;
+  for (int i = 0; i < 9; i++) k.alpha[i] = alpha_dev[i];
+  const uint32_t *p = static_cast<const uint32_t *>(prev);
+  uint32_t *nx = static_cast<uint32_t *>(next);
+  const uint64_t half = 1ull << leaf_log;
+  return with_field(field, [&](auto f) {
+    using F = decltype(f);
+    return build_tree(FoldLeaves<F>{p, nx, inv_pows, half, k}, leaf_log, heap, true, s);
+  });
 }
 
 hipError_t launch_merkle_from_hashes(const void *hashes, uint64_t count, int cap_log, uint32_t *heap, bool full,

@@ -157,33 +157,37 @@ int prove(zntt_ctx *c, int field, const zntt_fri_config *cfg, const zntt_transcr
   uint64_t *d_seeds = (uint64_t *)mem;
   uint8_t *d_out = mem + ebytes;
 
-  const Elem half_inv = f_inv(field, f_from_u64(field, 2));
-  const std::array<uint32_t, 9> half_dev = f_to_dev(field, half_inv);
   // Domain generators: g_r (order 2^(ld - r)) is the primitive root of that
   // order, and g_{r+1} = g_r^2 (Domain.init, root.zig:133-138), so one root
   // and one inversion serve every round.
   Elem ginv = f_inv(field, f_primitive_root(field, (int)ld));
+  // Round r commits layer r (root.zig:378-384), absorbs its root and
+  // squeezes alpha (:387-389), then folds layer r into layer r + 1
+  // (:392-403).  The fold runs inside the launch that hashes layer r + 1's
+  // leaves, so each layer after the first is committed by the launch that
+  // produces it; only the last fold, whose layer is not committed, runs
+  // alone.  The fold reads g^-j from the table of powers of the inverse of
+  // the layer's domain generator.
+  hipError_t e = zntt::launch_merkle_layer(field, layer[0], (int)ld - 1, (uint32_t *)tree[0], true, s);
+  if (e != hipSuccess) return hip_status(e);
   for (size_t r = 0; r < rounds; r++) {
-    const int log_cur = (int)(ld - r);
     const size_t half = n >> (r + 1);
-    // Commit (root.zig:378-384).
-    hipError_t e = zntt::launch_merkle_layer(field, layer[r], log_cur - 1, (uint32_t *)tree[r], true, s);
     uint8_t *root = proof->roots + 32 * r;
-    if (e == hipSuccess) e = hipMemcpyAsync(pin_out, tree[r] + 32, 32, hipMemcpyDeviceToHost, s);
+    e = hipMemcpyAsync(pin_out, tree[r] + 32, 32, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return hip_status(e);
     memcpy(root, pin_out, 32);
-    // Absorb the root, squeeze alpha (:387-389).
     ops->absorb_bytes(tr, root, 32);
     const Elem alpha = challenge_field(ops, tr, field);
-    // Fold (:392-403) with g^-j from the table of powers of the inverse of
-    // the layer's domain generator (Domain.init: omega^(2^(adicity - k)),
-    // the primitive root of order 2^k).
     const uint32_t *tab = nullptr;
     if ((st = coset_table(c, field, ginv, half, s, &tab))) return st;
     ginv = f_mul(field, ginv, ginv);
-    const std::array<uint32_t, 9> ah_dev = f_to_dev(field, f_mul(field, alpha, half_inv));
-    e = zntt::launch_fri_fold(field, layer[r], layer[r + 1], half, tab, ah_dev.data(), half_dev.data(), s);
+    const std::array<uint32_t, 9> alpha_dev = f_to_dev(field, alpha);
+    if (r + 1 < rounds)
+      e = zntt::launch_merkle_fold(field, layer[r], layer[r + 1], (int)(ld - r) - 2, (uint32_t *)tree[r + 1], tab,
+                                   alpha_dev.data(), s);
+    else
+      e = zntt::launch_fri_fold(field, layer[r], layer[r + 1], half, tab, alpha_dev.data(), s);
     if (e != hipSuccess) return hip_status(e);
   }
 
@@ -198,7 +202,7 @@ int prove(zntt_ctx *c, int field, const zntt_fri_config *cfg, const zntt_transcr
     if ((st = power_table(c, field, ginv, lf, s, &inv))) return st;  // g_R^-1, R = ld - lf
     const std::array<uint32_t, 9> scale = f_to_dev(field, f_inv(field, f_from_u64(field, m)));
     if ((st = run_device(c, field, lf, layer[rounds], inv->table, nullptr, &scale, s))) return st;
-    hipError_t e = hipMemcpyAsync(pin_out, layer[rounds], rl * ub, hipMemcpyDeviceToHost, s);
+    e = hipMemcpyAsync(pin_out, layer[rounds], rl * ub, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return hip_status(e);
     for (size_t i = 0; i < rl; i++) {
@@ -223,7 +227,7 @@ int prove(zntt_ctx *c, int field, const zntt_fri_config *cfg, const zntt_transcr
   oa.rounds = (uint32_t)rounds;
   oa.log_domain = (uint32_t)ld;
   oa.elem_words = (uint32_t)(ub / 4);
-  hipError_t e = hipMemcpyAsync(d_seeds, seeds, q * 8, hipMemcpyHostToDevice, s);
+  e = hipMemcpyAsync(d_seeds, seeds, q * 8, hipMemcpyHostToDevice, s);
   if (e == hipSuccess)
     e = zntt::launch_fri_open(oa, d_seeds, q, (uint32_t *)d_out, (uint32_t *)(d_out + nvals * ub), s);
   if (e == hipSuccess) e = hipMemcpyAsync(pin_out, d_out, obytes, hipMemcpyDeviceToHost, s);
