@@ -547,6 +547,44 @@ def run_fri(args, Z, ctx):
     return line
 
 
+def run_m31_vec8(args, Z, ctx):
+    """Eight-lane M31 transform (Vec8NttM31, libs/field/src/lib.zig:52-106):
+    8 interleaved transforms of 2^log_n values resident in HBM, root p - 1
+    (the field's only nontrivial 2-power root; the DAG is computed for it at
+    every size, as the reference does)."""
+    import numpy as np
+    import torch
+    L = min(args.log_n, 22)
+    n = 1 << L
+    P = Z.M31_MODULUS
+    data = np.random.default_rng(5).integers(0, P, size=8 * n, dtype=np.uint64)
+    t = torch.from_numpy(data.view(np.int64).copy()).cuda()
+    stream = torch.cuda.current_stream()
+    ms = timed(lambda: ctx.vec8_m31_device(t.data_ptr(), L, P - 1, stream=stream.cuda_stream), args.steps,
+               args.warmup, stream)
+    moved = 8 * n * 8 * 2 * 2 + 8 * n * 4 * 2  # gather + scatter (u64 r/w), the transforms' own passes (u32 dev)
+    roofline = {"bound": "hbm", "achieved": 8 * n * 8 * 2 / (ms * 1e-3) / 1e9, "peak": HBM_PEAK / 1e9,
+                "unit": "GB/s", "frac": 8 * n * 8 * 2 / (ms * 1e-3) / HBM_PEAK, "traffic": None}
+    cpu = None
+    if not args.skip_cpu:
+        import oracle as O
+        m = 16
+        d = data[: 8 << m].copy()
+        t1 = time.time()
+        assert O.vec8_m31(d, m, P - 1) == O.OK
+        cs = time.time() - t1
+        cpu = {"value": (8 << m) / cs / 1e6, "unit": "Melem/s", "cores": 1, "kind": "port",
+               "sample": f"8 x 2^{m} values, C oracle restating Vec8NttM31, 1 thread, {cs:.2f} s"}
+    line = base_line(f"Melem/s M31 eight-lane NTT 8 x 2^{L}", 8 * n / (ms * 1e-3) / 1e6, "Melem/s", args, ms,
+                     {"workload": f"Vec8NttM31: 8 interleaved M31 transforms of 2^{L}, resident in HBM",
+                      "field": "m31", "evaluations": 8 * n},
+                     "synthetic: uniform M31 values, seed 5", "u32")
+    line.update({"roofline": roofline, "cpu_baseline": cpu,
+                 "extras": {"bytes_moved_estimate": moved,
+                            "roofline_note": "achieved counts one read and one write of the u64 data"}})
+    return line
+
+
 def serve_muls(arrivals):
     """Algorithmic 32-bit multiplies of a schedule: per request (N/2) log2 N
     butterfly products plus 2^k coset products, N = 2^(k+b)."""
@@ -634,7 +672,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="fr-ntt",
                     choices=["fr-ntt", "gl-batch", "bb-batch", "kb-batch", "fr-uneven", "serve", "ref-2p20",
-                             "poly-mul", "merkle", "fri"])
+                             "poly-mul", "merkle", "fri", "m31-vec8"])
     ap.add_argument("--log-n", type=int, default=24)
     ap.add_argument("--cpu-log-n", type=int, default=24, help="size of the CPU-baseline sample")
     ap.add_argument("--skip-cpu", action="store_true")
@@ -668,6 +706,8 @@ def main() -> int:
         line = run_merkle(args, Z, ctx)
     elif args.workload == "fri":
         line = run_fri(args, Z, ctx)
+    elif args.workload == "m31-vec8":
+        line = run_m31_vec8(args, Z, ctx)
     else:
         line = run_serve(args, Z, ctx)
     print(json.dumps(line), flush=True)

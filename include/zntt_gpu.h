@@ -465,6 +465,33 @@ void zntt_transcript_challenge_bytes(void *t, uint8_t *out, size_t len);
  * exposed for checks. */
 void zntt_blake3(const void *data, size_t len, uint8_t *out);
 
+/* ------------------------------------------------------------------------ */
+/* Eight-lane Mersenne-31 transform (Vec8NttM31, libs/field/src/lib.zig:     */
+/* 39-137).  `data` holds eight interleaved transforms of 2^log_n M31 values, */
+/* data[i * 8 + lane], each value a canonical u64 (the reference's M31,      */
+/* p = 2^31 - 1).  Each lane gets zig-ntt's radix-2 DIT DAG with `root`      */
+/* (M31's two-adicity is 1, so for log_n >= 2 no root is primitive; the DAG  */
+/* is computed for whatever root is given, as the reference does).           */
+/* ------------------------------------------------------------------------ */
+
+/* Largest log_n the device path implements (8 x 2^25 values = 2 GiB). */
+#define ZNTT_VEC8_MAX_LOG_N 25
+
+/* nttVec8M31Checked(data, log_n, root) -- lib.zig:52-106 (host buffer, in
+ * place, synchronous).  ZNTT_INVALID_LENGTH when log_n >= 61 or
+ * len != 8 * 2^log_n, before `data` is touched (lib.zig:53-55, 131-136);
+ * ZNTT_UNSUPPORTED beyond ZNTT_VEC8_MAX_LOG_N.  The reference's unchecked
+ * nttVec8M31 is the same call with the status ignored. */
+int zntt_ntt_vec8_m31(zntt_ctx *ctx, void *data, size_t len, size_t log_n, const void *root);
+
+/* inttVec8M31Checked -- lib.zig:114-129: the forward DAG with root^-1 (zero
+ * for a zero root), then every value times (2^log_n)^-1. */
+int zntt_intt_vec8_m31(zntt_ctx *ctx, void *data, size_t len, size_t log_n, const void *root);
+
+/* The same on a device buffer of 8 * 2^log_n values, enqueued on `stream`;
+ * returns when the transform is done.  inverse: 0 forward, 1 inverse. */
+int zntt_vec8_m31_device(zntt_ctx *ctx, int inverse, void *data, size_t log_n, const void *root, void *stream);
+
 /* Blocks until every call enqueued on the context's own stream is done. */
 int zntt_ctx_synchronize(zntt_ctx *ctx);
 

@@ -25,6 +25,8 @@ FIELD_FR = 1
 FIELD_GL = 2
 FIELD_BB = 3
 FIELD_KB = 4
+FIELD_M31 = 5
+M31_MODULUS = 0x7FFFFFFF
 
 FR_MODULUS = 0x30644E72E131A029B85045B68181585D2833E84879B9709143E1F593F0000001
 GL_MODULUS = 0xFFFFFFFF00000001
@@ -100,6 +102,7 @@ def lib():
         L.oracle_tr_challenge_u64.restype = u64
         L.oracle_tr_challenge_field.argtypes = [vp, i32, vp]
         L.oracle_fri_prove.argtypes = [i32, vp, vp, sz, sz, sz, sz, sz, sz, vp, vp, vp, vp, vp]
+        L.oracle_vec8_m31.argtypes = [i32, vp, sz, sz, u64]
         L.oracle_init()
         _lib = L
     return _lib
@@ -385,3 +388,10 @@ def fri_prove(field: int, transcript: Transcript, evals: np.ndarray, log_domain:
     return st, {"roots": roots[:sz["roots"]].tobytes(), "residual": residual[:sz["residual"]].tobytes(),
                 "pair_index": [int(x) for x in pair[:num_queries]], "values": values[:sz["values"]].tobytes(),
                 "paths": paths[:sz["paths"]].tobytes()}
+
+
+def vec8_m31(data: np.ndarray, log_n: int, root: int, inverse: bool = False, length: int | None = None) -> int:
+    """Vec8NttM31 (in place) on interleaved u64 values data[i * 8 + lane]."""
+    assert data.dtype == np.uint64 and data.flags["C_CONTIGUOUS"]
+    n = data.shape[0] if length is None else length
+    return lib().oracle_vec8_m31(1 if inverse else 0, _ptr(data), n, log_n, root)

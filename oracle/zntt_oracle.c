@@ -15,8 +15,11 @@
  *       to/fromMontgomery   libs/field/src/montgomery.zig:398-405
  *       pow (CT ladder)     libs/field/src/field.zig:1265-1287
  *       Fr modulus          libs/curve/src/bn254.zig:67
- *   - small fields (modulus < 2^64, non-Mersenne: Goldilocks, BabyBear) as a
- *     canonical u64 (libs/field/src/field.zig:89-424, pow at :534-556)
+ *   - small fields (Goldilocks, BabyBear, KoalaBear, and M31 for the
+ *     eight-lane transform) as a canonical u64 with u128 `%` arithmetic
+ *     (libs/field/src/field.zig:89-424, pow at :534-556); the reference's
+ *     Mersenne path for M31 (:298-368) computes the same canonical residues
+ *   - the eight-lane M31 transform, Vec8NttM31 (libs/field/src/lib.zig:39-137)
  *   - roots of unity          libs/field/src/roots.zig:22-30, 149-157
  *   - the NTT itself          libs/ntt/src/root.zig:33-212 (all seven
  *     functions, with the same check order and the same butterfly DAG)
@@ -651,17 +654,19 @@ static uint64_t sf_draw(const sf_ctx *c, xo_t *x) {
 #define FIELD_GL 2
 #define FIELD_BB 3
 #define FIELD_KB 4
+#define FIELD_M31 5
 
 static int sf_for(int field, sf_ctx *c) {
   if (field == FIELD_GL) { *c = sf_make(0xFFFFFFFF00000001ull); return 1; }
   if (field == FIELD_BB) { *c = sf_make(0x78000001ull); return 1; }
   if (field == FIELD_KB) { *c = sf_make(0x7F000001ull); return 1; }
+  if (field == FIELD_M31) { *c = sf_make(0x7FFFFFFFull); return 1; }
   return 0;
 }
 
 int oracle_elem_size(int field) {
   if (field == FIELD_FR) return 32;
-  if (field == FIELD_GL || field == FIELD_BB || field == FIELD_KB) return 8;
+  if (field == FIELD_GL || field == FIELD_BB || field == FIELD_KB || field == FIELD_M31) return 8;
   return 0;
 }
 
@@ -1472,5 +1477,61 @@ int oracle_fri_prove(int field, oracle_tr *t, const void *evals, size_t n, size_
   for (size_t r = 0; r < rounds; r++) free(tree[r]);
   free(layer);
   free(tree);
+  return OR_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Vec8NttM31 (libs/field/src/lib.zig:52-137): eight transforms interleaved  */
+/* as data[i * 8 + lane] over M31, restated loop for loop: per-lane bit      */
+/* reversal, then stages s = 1..log_n, wm = root^(2^(log_n - s)), w stepped  */
+/* by w *= wm, butterflies (u + w v, u - w v) reduced to canonical values.   */
+/* ------------------------------------------------------------------------ */
+
+static int vec8_forward(const sf_ctx *c, uint64_t *data, size_t len, size_t log_n, uint64_t root) {
+  if (log_n >= 61) return OR_INVALID_LENGTH;  /* transformSize, lib.zig:131-136 */
+  const size_t n = (size_t)1 << log_n;
+  if (len != 8 * n) return OR_INVALID_LENGTH;
+  for (size_t lane = 0; lane < 8; lane++)
+    for (size_t i = 0; i < n; i++) {
+      size_t j = 0;
+      for (size_t k = 0; k < log_n; k++) j = (j << 1) | ((i >> k) & 1);
+      if (j > i) {
+        const uint64_t t = data[i * 8 + lane];
+        data[i * 8 + lane] = data[j * 8 + lane];
+        data[j * 8 + lane] = t;
+      }
+    }
+  const sf_t r = {root};
+  for (size_t s = 1; s <= log_n; s++) {
+    const size_t m = (size_t)1 << s, half_m = m >> 1;
+    const sf_t wm = sf_pow(c, r, (uint64_t)1 << (log_n - s));
+    for (size_t k = 0; k < n; k += m) {
+      sf_t w = sf_one(c);
+      for (size_t j = 0; j < half_m; j++) {
+        for (size_t l = 0; l < 8; l++) {
+          const sf_t u = {data[(k + j) * 8 + l]}, v = {data[(k + j + half_m) * 8 + l]};
+          const sf_t t = sf_mul(c, w, v);
+          data[(k + j) * 8 + l] = sf_add(c, u, t).v;
+          data[(k + j + half_m) * 8 + l] = sf_sub(c, u, t).v;
+        }
+        w = sf_mul(c, w, wm);
+      }
+    }
+  }
+  return OR_OK;
+}
+
+int oracle_vec8_m31(int inverse, uint64_t *data, size_t len, size_t log_n, uint64_t root) {
+  sf_ctx c;
+  sf_for(FIELD_M31, &c);
+  if (!inverse) return vec8_forward(&c, data, len, log_n, root);
+  const sf_t r = {root};
+  int st = vec8_forward(&c, data, len, log_n, sf_inv(&c, r).v);
+  if (st) return st;
+  const sf_t ninv = sf_inv(&c, sf_from_u64(&c, (uint64_t)1 << log_n));
+  for (size_t i = 0; i < len; i++) {
+    const sf_t x = {data[i]};
+    data[i] = sf_mul(&c, x, ninv).v;
+  }
   return OR_OK;
 }

@@ -6,7 +6,8 @@
 
 namespace zntt {
 
-enum FieldId { kFr = 1, kGl = 2, kBb = 3, kKb = 4 };
+// kM31 is internal: only the eight-lane M31 transform uses it.
+enum FieldId { kFr = 1, kGl = 2, kBb = 3, kKb = 4, kM31 = 5 };
 
 // Words (u32) per element in the kernels' two layouts.
 //  user: the reference's storage layout (Fr 8, Goldilocks 2)
@@ -110,6 +111,15 @@ struct FriOpenArgs {
 };
 hipError_t launch_fri_open(const FriOpenArgs &a, const uint64_t *e0, uint64_t q, uint32_t *vals, uint32_t *paths,
                            hipStream_t s);
+// Mersenne-31 (pass_m31.hip), for the eight-lane transform: one transform
+// of 2^L u64 values (tw: flat table from launch_m31_table; scale_dev: a plain
+// multiplier of every output, or null), the table of root's stage powers
+// (pows[k] = root^(2^k), k < L, plain u32), and gathering the eight
+// interleaved lanes (data[i * 8 + lane]) into lane-major order or back.
+hipError_t launch_transform_m31(int L, uint32_t *data, uint32_t *scratch, const uint32_t *tw,
+                                const uint32_t *scale_dev, hipStream_t stream);
+hipError_t launch_m31_table(int L, uint32_t *flat, const uint32_t *pows, hipStream_t s);
+hipError_t launch_m31_lanes(bool gather, const void *src, void *dst, uint64_t n, hipStream_t s);
 // Elementwise device field op (0 add, 1 sub, 2 mul) on user-layout arrays.
 hipError_t launch_field_op(int field, int op, size_t n, const uint32_t *a, const uint32_t *b, uint32_t *out,
                            hipStream_t stream);

@@ -104,7 +104,10 @@ EXPORTED = [
     "zntt_poly_mul", "zntt_poly_mul_device", "zntt_merkle_commit", "zntt_merkle_from_hashes",
     "zntt_fri_proof_sizes", "zntt_fri_prove", "zntt_transcript_create", "zntt_transcript_destroy",
     "zntt_transcript_absorb_bytes", "zntt_transcript_challenge_bytes", "zntt_blake3",
+    "zntt_ntt_vec8_m31", "zntt_intt_vec8_m31", "zntt_vec8_m31_device",
 ]
+
+M31_MODULUS = 0x7FFFFFFF
 
 BYTES_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint8), ctypes.c_size_t)
 
@@ -241,6 +244,9 @@ def _load():
     L.zntt_transcript_challenge_bytes.restype = None
     L.zntt_blake3.argtypes = [ctypes.c_char_p, sz, vp]
     L.zntt_blake3.restype = None
+    L.zntt_ntt_vec8_m31.argtypes = [vp, vp, sz, sz, vp]
+    L.zntt_intt_vec8_m31.argtypes = [vp, vp, sz, sz, vp]
+    L.zntt_vec8_m31_device.argtypes = [vp, i32, vp, sz, vp, vp]
     return L
 
 
@@ -496,6 +502,21 @@ class Context:
         out = {k: bytes(bufs[k])[:sz[k]] for k in sz}
         out["pair_index"] = [int(x) for x in pair]
         return out
+
+    def ntt_vec8_m31(self, data: np.ndarray, log_n: int, root: int, inverse: bool = False,
+                     length: int | None = None) -> None:
+        """Vec8NttM31 in place on interleaved u64 values data[i * 8 + lane]."""
+        assert data.dtype == np.uint64 and data.flags["C_CONTIGUOUS"]
+        r = np.array([root], dtype=np.uint64)
+        fn = lib.zntt_intt_vec8_m31 if inverse else lib.zntt_ntt_vec8_m31
+        check(fn(self._h, _ptr(data), data.shape[0] if length is None else length, log_n, _ptr(r)))
+
+    def vec8_m31_device(self, data_ptr: int, log_n: int, root: int, inverse: bool = False,
+                        stream: int | None = None) -> None:
+        if stream is None:
+            stream = lib.zntt_ctx_stream(self._h)
+        r = np.array([root], dtype=np.uint64)
+        check(lib.zntt_vec8_m31_device(self._h, 1 if inverse else 0, data_ptr, log_n, _ptr(r), stream))
 
     def lde(self, field: int, coeffs: np.ndarray, log_k: int, log_blowup: int, shift) -> np.ndarray:
         out = empty(field, 1 << (log_k + log_blowup))
