@@ -5,6 +5,7 @@
 // ntt_kernels.hip for the overall schedule.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <type_traits>
 #include <stdint.h>
 
 #include <algorithm>
@@ -149,8 +150,17 @@ __device__ __forceinline__ void stage_whole_tile(const PassArgs &a, uint32_t *ld
   }
 }
 
-// The four butterflies of radix-2 stage T inside a round of R stages over a
-// thread's eight elements (8 >> R sets of 2^R): butterfly j couples elements
+// Elements a thread holds in the register rounds of a field's large passes:
+// eight, except BN254 Fr, whose nine-limb values make eight a thread cost
+// about 240 VGPRs (two waves a SIMD); four halve that, and a 2^11 tile then
+// runs as 512 threads, so LDS and registers both allow four waves a SIMD.
+template <class F>
+constexpr int round_elems() {
+  return std::is_same<F, FrT>::value ? 4 : 8;
+}
+
+// The butterflies of radix-2 stage T inside a round of R stages over a
+// thread's N elements (N >> R sets of 2^R): butterfly j couples elements
 // u(j) and u(j) + 2^T of set set(j), with twiddle index j(j) within the stage.
 template <int R, int T>
 struct StageMap {
@@ -166,11 +176,11 @@ struct StageMap {
 
 // TWL: the pass's twiddle table is resident in LDS after the tile's planes
 // (a whole transform: 2^K entries, plane stride 2^K), not read from HBM.
-template <class F, int R, int l0, bool first_round, bool last_round, bool ODD, bool TWL = false>
-__device__ __forceinline__ void pass_round(typename F::E (&x)[8], const PassArgs &a, uint32_t *lds,
+template <class F, int R, int l0, bool first_round, bool last_round, bool ODD, bool TWL = false, int N = 8>
+__device__ __forceinline__ void pass_round(typename F::E (&x)[N], const PassArgs &a, uint32_t *lds,
                                            const TileCtx &c) {
   using E = typename F::E;
-  constexpr int SET = 1 << R, NSETS = 8 >> R;
+  constexpr int SET = 1 << R, NSETS = N >> R;
   int g[NSETS], base[NSETS];
   sfor<NSETS>([&](auto i) {
     const int slot = threadIdx.x + i * blockDim.x;
@@ -232,19 +242,19 @@ __device__ __forceinline__ void pass_round(typename F::E (&x)[8], const PassArgs
         const int mid = base[i] | (k << l0);
         return F::load_dev(a.coset, ((size_t)brev_bits(mid, c.K) << c.hi_bits) + c.c0 + g[i]);
       };
-      sfor<4>([&](auto h) {
+      sfor<N / 2>([&](auto h) {
         constexpr int e = 2 * decltype(h)::value;
         F::mul2(x[e], cos(e), x[e + 1], cos(e + 1));
       });
     }
   }
   // R radix-2 stages in registers; carry-normalise u at odd stages.  The
-  // four butterflies of a stage go out as two interleaved pairs.
+  // N/2 butterflies of a stage go out as interleaved pairs.
   sfor<R>([&](auto t) {
     constexpr int T = decltype(t)::value;
     constexpr bool norm = (((ODD ? 1 : 0) + l0 + T) & 1) != 0;
     using M = StageMap<R, T>;
-    sfor<2>([&](auto h) {
+    sfor<N / 4>([&](auto h) {
       constexpr int j0 = 2 * decltype(h)::value, j1 = j0 + 1;
       auto both = [&] {
         F::template bfly2<norm>(x[M::u(j0)], x[M::u(j0) + (1 << T)],
@@ -279,7 +289,7 @@ __device__ __forceinline__ void pass_round(typename F::E (&x)[8], const PassArgs
   if constexpr (last_round) {
     if (a.last && a.has_scale) {
       const E scale = F::from_words(a.scale);
-      sfor<4>([&](auto h) {
+      sfor<N / 2>([&](auto h) {
         constexpr int e = 2 * decltype(h)::value;
         F::mul2(x[e], scale, x[e + 1], scale);
       });
@@ -305,23 +315,25 @@ __device__ __forceinline__ void pass_round(typename F::E (&x)[8], const PassArgs
   });
 }
 
-// Rounds of a K-stage pass: the first takes K - 3*(ceil(K/3) - 1) stages, the
-// rest three each; unrolled at compile time so `x` stays in registers.
-template <class F, int K, int RI, bool ODD, bool TWL = false>
-__device__ __forceinline__ void pass_rounds(typename F::E (&x)[8], const PassArgs &a, uint32_t *lds,
+// Rounds of a K-stage pass over N = 2^LN elements a thread: the first takes
+// K - LN*(ceil(K/LN) - 1) stages, the rest LN each; unrolled at compile time
+// so `x` stays in registers.
+template <class F, int K, int RI, bool ODD, bool TWL = false, int N = 8>
+__device__ __forceinline__ void pass_rounds(typename F::E (&x)[N], const PassArgs &a, uint32_t *lds,
                                             const TileCtx &c) {
-  constexpr int NR = (K + 2) / 3;
-  constexpr int R0 = K - 3 * (NR - 1);
-  constexpr int R = RI == 0 ? R0 : 3;
-  constexpr int L0 = RI == 0 ? 0 : R0 + 3 * (RI - 1);
-  pass_round<F, R, L0, RI == 0, RI == NR - 1, ODD, TWL>(x, a, lds, c);
+  constexpr int LN = N == 8 ? 3 : 2;
+  constexpr int NR = (K + LN - 1) / LN;
+  constexpr int R0 = K - LN * (NR - 1);
+  constexpr int R = RI == 0 ? R0 : LN;
+  constexpr int L0 = RI == 0 ? 0 : R0 + LN * (RI - 1);
+  pass_round<F, R, L0, RI == 0, RI == NR - 1, ODD, TWL, N>(x, a, lds, c);
   if constexpr (RI + 1 < NR) {
     __syncthreads();
-    pass_rounds<F, K, RI + 1, ODD, TWL>(x, a, lds, c);
+    pass_rounds<F, K, RI + 1, ODD, TWL, N>(x, a, lds, c);
   }
 }
 
-template <class F, int K, bool ODD>
+template <class F, int K, bool ODD, int N = 8>
 __global__ __launch_bounds__(512) void ntt_pass_r8_kernel(const PassArgs a) {
   using E = typename F::E;
   extern __shared__ uint32_t lds[];
@@ -341,8 +353,8 @@ __global__ __launch_bounds__(512) void ntt_pass_r8_kernel(const PassArgs a) {
     c.hi = w >> lbb;
     c.lo0 = (w & (((size_t)1 << lbb) - 1)) << c.lg;
   }
-  E x[8];
-  pass_rounds<F, K, 0, ODD>(x, a, lds, c);
+  E x[N];
+  pass_rounds<F, K, 0, ODD, false, N>(x, a, lds, c);
 }
 
 // Persistent launches claim items from ctr[0] and count finished workgroups
@@ -406,8 +418,9 @@ __global__ __launch_bounds__(512) void whole_batch_kernel(const BatchXform *xf, 
     c.lo_bits = 0;
     c.hi_bits = 0;
     c.hi = c.lo0 = c.c0 = 0;
-    E x[8];
-    pass_rounds<F, K, 0, true, TWL>(x, a, lds, c);
+    constexpr int N = round_elems<F>();
+    E x[N];
+    pass_rounds<F, K, 0, true, TWL, N>(x, a, lds, c);
     __syncthreads();  // LDS is reused by the next item
     if (threadIdx.x == 0) s_item = claimed;
     __syncthreads();
@@ -420,7 +433,7 @@ template <class F, int K, bool TWL>
 static hipError_t launch_whole_k(const BatchXform *xf, const BatchEntry *ent, uint32_t n, uint32_t *counter,
                                  const uint32_t *tw_all, hipStream_t s) {
   const size_t lds = (size_t)F::W * 4 << K << (TWL ? 1 : 0);
-  const unsigned threads = 1u << (K - 3);
+  const unsigned threads = (1u << K) / round_elems<F>();
   // Resident workgroups on the whole device, computed once.
   static std::once_flag once;
   static int grid = 0;
@@ -473,10 +486,11 @@ static hipError_t launch_whole_t(int K, const BatchXform *xf, const BatchEntry *
 // Pass lengths the planner produces for tiles of >= 512 elements: 5..12.
 template <class F>
 static hipError_t launch_r8(int K, bool odd, dim3 grid, dim3 block, size_t lds, hipStream_t s, const PassArgs &a) {
+  constexpr int N = round_elems<F>();
   switch (K * 2 + (odd ? 1 : 0)) {
-#define ZNTT_R8(k)                                                                                     \
-  case 2 * k: hipLaunchKernelGGL((ntt_pass_r8_kernel<F, k, false>), grid, block, lds, s, a); break; \
-  case 2 * k + 1: hipLaunchKernelGGL((ntt_pass_r8_kernel<F, k, true>), grid, block, lds, s, a); break;
+#define ZNTT_R8(k)                                                                                        \
+  case 2 * k: hipLaunchKernelGGL((ntt_pass_r8_kernel<F, k, false, N>), grid, block, lds, s, a); break; \
+  case 2 * k + 1: hipLaunchKernelGGL((ntt_pass_r8_kernel<F, k, true, N>), grid, block, lds, s, a); break;
     ZNTT_R8(5) ZNTT_R8(6) ZNTT_R8(7) ZNTT_R8(8) ZNTT_R8(9) ZNTT_R8(10) ZNTT_R8(11) ZNTT_R8(12)
 #undef ZNTT_R8
     default: return hipErrorInvalidValue;
@@ -486,9 +500,10 @@ static hipError_t launch_r8(int K, bool odd, dim3 grid, dim3 block, size_t lds, 
 
 template <class F>
 static hipError_t set_r8_lds(size_t bytes) {
+  constexpr int N = round_elems<F>();
 #define ZNTT_R8A(k, o)                                                                                       \
   {                                                                                                          \
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&ntt_pass_r8_kernel<F, k, o>),        \
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&ntt_pass_r8_kernel<F, k, o, N>),     \
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);              \
     if (e != hipSuccess) return e;                                                                           \
   }
@@ -524,9 +539,10 @@ static hipError_t run_passes_t(int field, int L, uint32_t *data, uint32_t *scrat
     const size_t tile = (size_t)1 << (a.K + a.log_g);
     const size_t blocks = ((size_t)1 << L) / tile;
     const size_t lds = (size_t)F::W * 4 * tile;
-    if (tile >= 512 && a.K >= 5) {  // eight elements per thread, register rounds
+    constexpr int N = round_elems<F>();
+    if (tile >= 512 && a.K >= 5 && tile / N <= 512) {  // N elements per thread, register rounds
       hipError_t e =
-          launch_r8<F>(a.K, (a.s0 & 1) != 0, dim3((unsigned)blocks), dim3((unsigned)(tile / 8)), lds, stream, a);
+          launch_r8<F>(a.K, (a.s0 & 1) != 0, dim3((unsigned)blocks), dim3((unsigned)(tile / N)), lds, stream, a);
       if (e != hipSuccess) return e;
     } else {
       const int threads = (int)((tile / 2) < 64 ? 64 : tile / 2);
