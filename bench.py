@@ -183,7 +183,7 @@ def run_small_batch(args, Z, ctx, field):
                 "frac": achieved / HBM_PEAK, "traffic": None}
     cpu = None
     if not args.skip_cpu:
-        sample = 512
+        sample = count  # the whole batch, ~1.6 s of one core
         cx = x[:sample << L].copy()
         roots = np.zeros(33, dtype=np.uint64)
         roots[L] = root[0]
@@ -233,7 +233,7 @@ def run_gl_batch(args, Z, ctx):
     cpu = None
     if not args.skip_cpu:
         import oracle as O
-        sample = 512  # transforms, ~10 s of one core
+        sample = 4096  # the whole batch, ~1.6 s of one core
         cx = x[:sample << L].copy()
         roots = np.zeros(33, dtype=np.uint64)
         roots[L] = root[0]
@@ -568,7 +568,7 @@ def run_m31_vec8(args, Z, ctx):
     cpu = None
     if not args.skip_cpu:
         import oracle as O
-        m = 16
+        m = L  # the same 8 x 2^L values, ~3 s of one core
         d = data[: 8 << m].copy()
         t1 = time.time()
         assert O.vec8_m31(d, m, P - 1) == O.OK
