@@ -98,6 +98,8 @@ typedef struct zntt_ctx zntt_ctx;
 
 /* hbm_budget_bytes: cap on device memory the context may hold (0 = 64 GiB). */
 int zntt_ctx_create(zntt_ctx **out, int device, uint64_t hbm_budget_bytes);
+/* Every batch and server made on `ctx` must be destroyed first: they use the
+ * context (its lock, stream and twiddle cache) until they are. */
 void zntt_ctx_destroy(zntt_ctx *ctx);
 /* Device bytes currently held (scratch + twiddle cache + staging). */
 uint64_t zntt_ctx_bytes_in_use(const zntt_ctx *ctx);

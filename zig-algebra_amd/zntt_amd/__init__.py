@@ -14,6 +14,7 @@ from __future__ import annotations
 
 import ctypes
 import os
+import weakref
 
 import numpy as np
 
@@ -370,19 +371,29 @@ def fri_proof_sizes(field: int, log_domain: int, log_initial_degree: int, log_fi
 
 
 class Context:
-    """One GPU: a HIP stream, scratch memory and the device twiddle cache."""
+    """One GPU: a HIP stream, scratch memory and the device twiddle cache.
+
+    Batches and servers made on a context hold it (zntt_ctx_destroy requires
+    them destroyed first), so close() closes those still open before it
+    releases the context."""
 
     def __init__(self, device: int = 0, hbm_budget_bytes: int = 0):
         h = ctypes.c_void_p()
         check(lib.zntt_ctx_create(ctypes.byref(h), device, hbm_budget_bytes))
         self._h = h
+        self._children = weakref.WeakSet()
 
     @property
     def handle(self):
         return self._h
 
+    def _adopt(self, child) -> None:
+        self._children.add(child)
+
     def close(self):
         if self._h:
+            for child in list(self._children):
+                child.close()
             lib.zntt_ctx_destroy(self._h)
             self._h = None
 
@@ -588,6 +599,7 @@ class Batch:
         h = ctypes.c_void_p()
         check(lib.zntt_batch_create(ctx.handle, arr, len(transforms), ctypes.byref(h)))
         self._h = h
+        ctx._adopt(self)
 
     def execute(self, stream: int | None = None) -> None:
         if stream is None:
@@ -668,6 +680,8 @@ class Server:
         h = ctypes.c_void_p()
         check(lib.zntt_server_create(None if ctx is None else ctx.handle, ctypes.byref(cfg), ctypes.byref(h)))
         self._h = h
+        if ctx is not None:
+            ctx._adopt(self)
         self._shift_keep = {}
 
     def submit(self, log_n: int, log_blowup: int, shift, coeffs_ptr: int | None = None, seed: int = 0,
