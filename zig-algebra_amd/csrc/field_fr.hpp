@@ -21,13 +21,16 @@
 //
 // Lazy reduction.  Values are not reduced modulo r between butterflies:
 //   * mont(a, b) < a*b/2^261 + r < 2r whenever a < 2^261 and b < r;
-//   * the butterfly writes u + t and u + 4r - t, so a transform of 2^L points
-//     keeps every value below r + 4r*L < 2^261 for L <= 28;
+//   * the butterfly writes u + t and u + 4r - t, so each stage adds at most
+//     4r to the bound.  Inputs are below 2r (a coset pre-scale is a
+//     product); the first two stages' twiddle-1 butterflies give at most 12r
+//     after stage 2 (u + 8r - v there), so a transform of 2^L points keeps
+//     every value below 4r(L + 1) < 2^261 for L <= 28;
 //   * limbs of the sum/difference may exceed 29 bits (up to 1.5 * 2^30 after
 //     one stage); mont() accepts a-limbs below 3 * 2^30 (the column bound), so
 //     only the `u` input of each butterfly is carry-normalised first.
 // The last pass reduces each value fully (quotient estimate from the top limb,
-// then conditional subtractions) and repacks it into the 8-word layout, so
+// then one conditional subtraction) and repacks it into the 8-word layout, so
 // results are canonical and bit-identical to the reference's.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -46,7 +49,8 @@ constexpr uint32_t kM29 = 0x1FFFFFFFu;
                      0x2b6d0301u, 0x229b8503u, 0x397098cfu, 0x00c19138u}
 // -r^-1 mod 2^29
 constexpr uint32_t kFrN0 = 0x0FFFFFFFu;
-// floor(2^40 / (r_8 + 1)) with r_8 = r >> 232 = 0x30644e, for the quotient estimate.
+// Quotient-estimate multiplier for fr_canon: just below 2^40 / (r_8 + 1) =
+// 346695.2, with r_8 = r >> 232 = 0x30644e.
 constexpr uint32_t kFrQMagic = 346694u;
 
 struct Fr9 {
@@ -227,6 +231,29 @@ __device__ __forceinline__ void fr_butterfly_one(Fr9 &u, Fr9 &v) {
   }
 }
 
+// 8r in a redundant radix-2^29 form with limbs 0..7 in [2^29 - 1, 2^30) and
+// top limb 8r_8 + 2, above the top limb of any value below 4r.
+#define ZNTT_FR_C8R {0x20000008u, 0x387d64fbu, 0x32e12286u, 0x3e84879au, 0x2c2e9418u, \
+                     0x36da0604u, 0x25370a07u, 0x32e1319fu, 0x01832272u}
+
+// Butterfly with twiddle 1 whose v is a sum of two loaded or multiplied
+// values (value < 4r, limbs < 2^30): the second stage's j = 0 butterflies.
+// v is carry-normalised first; 4r's redundant top limb can fall one short of
+// such a v's, so the difference is u + 8r - v.  (Cheaper than a product by
+// the twiddle 1: a normalisation instead of a Montgomery multiply.)
+template <bool NORM = true>
+__device__ __forceinline__ void fr_butterfly_one_c8(Fr9 &u, Fr9 &v) {
+  constexpr uint32_t C[9] = ZNTT_FR_C8R;
+  fr_norm(v);
+  if (NORM) fr_norm(u);
+#pragma unroll
+  for (int i = 0; i < 9; i++) {
+    const uint32_t ui = u.v[i], ti = v.v[i];
+    u.v[i] = ui + ti;
+    v.v[i] = (ui + C[i]) - ti;
+  }
+}
+
 // x <- x - r if x >= r (x carry-normalised, limbs < 2^29, value < 2^261).
 __device__ __forceinline__ void fr_csub_r(Fr9 &x) {
   constexpr uint32_t P[9] = ZNTT_FR_P29;
@@ -245,11 +272,20 @@ __device__ __forceinline__ void fr_csub_r(Fr9 &x) {
 }
 
 // Full reduction to the canonical representative in [0, r).
-// Requires value < 2^261 (guaranteed by the lazy-reduction bound).
+// Requires value < 2^261 (guaranteed by the lazy-reduction bound), so that
+// the normalised top limb x8 = floor(x / 2^232) is below 2^29.
+//
+// Quotient estimate q = floor(x8 * M / 2^40), M = kFrQMagic:
+//   * q <= floor(x / r): M * (r8 + 1) < 2^40 (r8 = floor(r / 2^232)), so
+//     q <= x8 / (r8 + 1) < x8 * 2^232 / r <= x / r;
+//   * floor(x / r) - q <= 1: checked exactly for every x8 < 2^29 at each
+//     point where floor(x / r) steps up (the largest x of a top limb x8 has
+//     floor(x / r) = floor(((x8 + 1) 2^232 - 1) / r), and q is
+//     nondecreasing in x8).
+// So x - q r lies in [0, 2r) and one conditional subtraction finishes.
 __device__ __forceinline__ void fr_canon(Fr9 &x) {
   constexpr uint32_t P[9] = ZNTT_FR_P29;
   fr_norm(x);
-  // q <= floor(x / r) and floor(x / r) - q <= 2 (see file comment).
   const uint32_t q = (uint32_t)(((uint64_t)x.v[8] * kFrQMagic) >> 40);
   int64_t acc = 0;
 #pragma unroll
@@ -259,7 +295,6 @@ __device__ __forceinline__ void fr_canon(Fr9 &x) {
     acc >>= 29;  // arithmetic shift: signed carry
   }
   x.v[8] = (uint32_t)(x.v[8] | ((uint32_t)acc << 29));  // acc == 0 here: x - q r >= 0
-  fr_csub_r(x);
   fr_csub_r(x);
 }
 

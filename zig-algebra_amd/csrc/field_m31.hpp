@@ -70,7 +70,7 @@ struct M31T {
     A::butterfly(u0, v0, w0);
     A::butterfly(u1, v1, w1);
   }
-  template <bool NORM = true>
+  template <bool NORM = true, bool LATE = false>
   __device__ static void bfly_one(E &u, E &v) {
     A::butterfly_one(u, v);
   }

@@ -82,13 +82,17 @@ struct FrT {
   __device__ static void bfly2(E &u0, E &v0, const E &w0, E &u1, E &v1, const E &w1) {
     fr_butterfly2<NORM>(u0, v0, w0, u1, v1, w1);
   }
-  // Twiddle 1.  Only where v is a loaded or multiplied value: the first stage
-  // of a transform (stage T of its first round).
-  template <bool NORM = true>
+  // Twiddle 1 in the first two stages of a transform (stages T = 0, 1 of its
+  // first round).  At T = 0, v is a loaded or multiplied value; at T = 1 it
+  // is a first-stage sum of two of them (LATE).
+  template <bool NORM = true, bool LATE = false>
   __device__ static void bfly_one(E &u, E &v) {
-    fr_butterfly_one<NORM>(u, v);
+    if constexpr (LATE)
+      fr_butterfly_one_c8<NORM>(u, v);
+    else
+      fr_butterfly_one<NORM>(u, v);
   }
-  static constexpr bool one_ok(int T) { return T == 0; }
+  static constexpr bool one_ok(int T) { return T <= 1; }
   // Canonical dev-form constant (for tables).
   __device__ static E canon(E x) {
     fr_canon(x);
@@ -142,7 +146,7 @@ struct GlT {
     gl_butterfly(u0, v0, w0);
     gl_butterfly(u1, v1, w1);
   }
-  template <bool NORM = true>
+  template <bool NORM = true, bool LATE = false>
   __device__ static void bfly_one(E &u, E &v) {
     gl_butterfly_one(u, v);
   }
@@ -188,7 +192,7 @@ struct F31T {
     A::butterfly(u0, v0, w0);
     A::butterfly(u1, v1, w1);
   }
-  template <bool NORM = true>
+  template <bool NORM = true, bool LATE = false>
   __device__ static void bfly_one(E &u, E &v) {
     A::butterfly_one(u, v);
   }

@@ -268,12 +268,12 @@ __device__ __forceinline__ void pass_round(typename F::E (&x)[N], const PassArgs
       if constexpr (one0 || one1) {
         if (a.first && a.tw_one) {
           if constexpr (one0)
-            F::template bfly_one<norm>(x[M::u(j0)], x[M::u(j0) + (1 << T)]);
+            F::template bfly_one<norm, (T > 0)>(x[M::u(j0)], x[M::u(j0) + (1 << T)]);
           else
             F::template bfly<norm>(x[M::u(j0)], x[M::u(j0) + (1 << T)],
                                    tw[M::set(j0) * (SET - 1) + (1 << T) - 1 + M::j(j0)]);
           if constexpr (one1)
-            F::template bfly_one<norm>(x[M::u(j1)], x[M::u(j1) + (1 << T)]);
+            F::template bfly_one<norm, (T > 0)>(x[M::u(j1)], x[M::u(j1) + (1 << T)]);
           else
             F::template bfly<norm>(x[M::u(j1)], x[M::u(j1) + (1 << T)],
                                    tw[M::set(j1) * (SET - 1) + (1 << T) - 1 + M::j(j1)]);
