@@ -77,6 +77,40 @@ def timed(step, steps, warmup, stream):
     return start.elapsed_time(end) / steps
 
 
+def cpu_cores() -> int:
+    """Host threads for the all-cores CPU baselines: the host's cores, at most
+    16 (one GPU's share of a box's CPUs)."""
+    return max(1, min(os.cpu_count() or 1, 16))
+
+
+def cpu_pool(work, items, cores, budget_s=10.0):
+    """Runs work(item) -> elements over `items` on `cores` threads until the
+    items or the time budget run out (the oracle's C calls release the GIL,
+    so the threads run in parallel).  Returns (items done, elements, seconds)."""
+    import threading
+    from concurrent.futures import ThreadPoolExecutor
+    lock, it, state = threading.Lock(), iter(items), {"done": 0, "elems": 0}
+    t0 = time.time()
+
+    def worker():
+        while True:
+            with lock:
+                if time.time() - t0 > budget_s:
+                    return
+                x = next(it, None)
+            if x is None:
+                return
+            e = work(x)
+            with lock:
+                state["done"] += 1
+                state["elems"] += e
+
+    with ThreadPoolExecutor(cores) as ex:
+        for f in [ex.submit(worker) for _ in range(cores)]:
+            f.result()
+    return state["done"], state["elems"], time.time() - t0
+
+
 def base_line(metric, value, unit, args, ms_step, config, data, dtype):
     return {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
@@ -181,7 +215,7 @@ def run_small_batch(args, Z, ctx, field):
     mad, copy = ctx.probe_peaks()
     roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK, "traffic": None}
-    cpu = None
+    cpu = cpu_all = None
     if not args.skip_cpu:
         sample = count  # the whole batch, ~1.6 s of one core
         cx = x[:sample << L].copy()
@@ -193,13 +227,21 @@ def run_small_batch(args, Z, ctx, field):
         cs = time.time() - t1
         cpu = {"value": (sample << L) / cs / 1e6, "unit": "Melem/s", "cores": 1, "kind": "port",
                "sample": f"{sample} of the 4096 transforms, C oracle, 1 thread, {cs:.1f} s"}
+        cores = cpu_cores()
+        cx = x[:sample << L].copy()
+        t1 = time.time()
+        O.ntt_batch_mt(field, cx, offs, np.full(sample, L, dtype=np.uint64), roots, cores)
+        cs = time.time() - t1
+        cpu_all = {"value": (sample << L) / cs / 1e6, "unit": "Melem/s", "cores": cores, "kind": "port",
+                   "sample": f"all 4096 transforms, C oracle, {cores} threads, {cs:.2f} s"}
     line = base_line(f"Melem/s {name} batch 4096 x 2^12 NTT", n / (ms * 1e-3) / 1e6, "Melem/s", args, ms,
                      {"workload": f"4096 independent {name} forward NTTs of 2^12, one persistent launch",
                       "field": name.lower(), "count": count, "log_n": L, "elements": n},
                      f"synthetic: uniform {name} in [0, p), seed 2", "u32")
     line.update({"roofline": roofline, "cpu_baseline": cpu,
                  "extras": {"batch_shape": b.shape(), "autotune_ms": tuned_ms, "bit_exact_sample_vs_oracle": ok,
-                            "measured_copy_GB_s": copy / 1e9, "frac_of_measured_copy": achieved / copy}})
+                            "measured_copy_GB_s": copy / 1e9, "frac_of_measured_copy": achieved / copy,
+                            "cpu_all_cores": cpu_all}})
     b.close()
     return line
 
@@ -230,7 +272,7 @@ def run_gl_batch(args, Z, ctx):
     achieved = alg_bytes / (ms * 1e-3)
     roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK, "traffic": read_traffic(GL_TRAFFIC_FILE)}
-    cpu = None
+    cpu = cpu_all = None
     if not args.skip_cpu:
         import oracle as O
         sample = 4096  # the whole batch, ~1.6 s of one core
@@ -243,6 +285,13 @@ def run_gl_batch(args, Z, ctx):
         cs = time.time() - t1
         cpu = {"value": (sample << L) / cs / 1e6, "unit": "Melem/s", "cores": 1, "kind": "port",
                "sample": f"{sample} of the 4096 transforms, C oracle, 1 thread, {cs:.1f} s"}
+        cores = cpu_cores()
+        cx = x[:sample << L].copy()
+        t1 = time.time()
+        O.ntt_batch_mt(O.FIELD_GL, cx, offs, np.full(sample, L, dtype=np.uint64), roots, cores)
+        cs = time.time() - t1
+        cpu_all = {"value": (sample << L) / cs / 1e6, "unit": "Melem/s", "cores": cores, "kind": "port",
+                   "sample": f"all 4096 transforms, C oracle, {cores} threads, {cs:.2f} s"}
     line = base_line("Melem/s Goldilocks batch 4096 x 2^12 NTT", n / (ms * 1e-3) / 1e6, "Melem/s", args, ms,
                      {"workload": "4096 independent Goldilocks forward NTTs of 2^12, one persistent launch",
                       "field": "goldilocks", "count": count, "log_n": L, "elements": n},
@@ -253,7 +302,7 @@ def run_gl_batch(args, Z, ctx):
                             "speedup_vs_per_call": per_call / ms, "bit_exact_vs_oracle_digest": bit_exact,
                             "butterflies": count * (1 << (L - 1)) * L,
                             "measured_copy_GB_s": copy / 1e9, "frac_of_measured_copy": achieved / copy,
-                            "measured_mad_peak_Tmul_s": mad / 1e12}})
+                            "measured_mad_peak_Tmul_s": mad / 1e12, "cpu_all_cores": cpu_all}})
     b.close()
     return line
 
@@ -287,7 +336,7 @@ def run_fr_uneven(args, Z, ctx):
     achieved = muls / (ms * 1e-3)
     roofline = {"bound": "valu", "achieved": achieved / 1e12, "peak": VALU_MUL_PEAK / 1e12, "unit": "Tmul32/s",
                 "frac": achieved / VALU_MUL_PEAK, "traffic": None}
-    cpu = None
+    cpu = cpu_all = None
     if not args.skip_cpu:
         import oracle as O
         done, elems, t1 = 0, 0, time.time()
@@ -304,6 +353,20 @@ def run_fr_uneven(args, Z, ctx):
         cs = time.time() - t1
         cpu = {"value": elems / cs / 1e6, "unit": "Melem/s", "cores": 1, "kind": "port",
                "sample": f"the first {done} of the 512 transforms, C oracle, 1 thread, {cs:.1f} s"}
+
+        def one(i):
+            L, d = spec[i]
+            cx = O.gen_uniform(O.FIELD_FR, W.item_seed(3, i), 1 << L)
+            r = O.primitive_root(O.FIELD_FR, L)[1]
+            st = (O.ntt, O.intt)[d](O.FIELD_FR, cx, L, r) if d < 2 else O.coset_ntt(O.FIELD_FR, cx, L, r, g5)
+            assert st == O.OK
+            return 1 << L
+
+        cores = cpu_cores()
+        done, elems, cs = cpu_pool(one, range(len(spec)), cores)
+        cpu_all = {"value": elems / cs / 1e6, "unit": "Melem/s", "cores": cores, "kind": "port",
+                   "sample": f"the first {done} of the 512 transforms, one per thread, C oracle, {cores} threads, "
+                             f"{cs:.1f} s"}
     line = base_line("Melem/s BN254 Fr uneven batch (512 x 2^10..2^20, mixed directions)",
                      n_total / (ms * 1e-3) / 1e6, "Melem/s", args, ms,
                      {"workload": "512 BN254 Fr transforms, log2 size uniform 10..20, forward/inverse/"
@@ -313,7 +376,7 @@ def run_fr_uneven(args, Z, ctx):
     line.update({"roofline": roofline, "cpu_baseline": cpu,
                  "extras": {"batch_shape": b.shape(), "autotune_ms": tuned_ms, "per_call_ms": pc_ms,
                             "per_call_melem_s": n_total / (pc_ms * 1e-3) / 1e6,
-                            "speedup_vs_per_call": pc_ms / ms}})
+                            "speedup_vs_per_call": pc_ms / ms, "cpu_all_cores": cpu_all}})
     b.close()
     return line
 
@@ -623,7 +686,7 @@ def run_serve(args, Z, ctx):
     achieved = main_run["muls"] / (main_run["busy_ms"] * 1e-3)
     roofline = {"bound": "valu", "achieved": achieved / 1e12, "peak": VALU_MUL_PEAK / 1e12, "unit": "Tmul32/s",
                 "frac": achieved / VALU_MUL_PEAK, "traffic": None}
-    cpu = None
+    cpu = cpu_all = None
     if not args.skip_cpu:
         import oracle as O
         arrivals = W.serve_arrivals(rate=args.rate, duration=args.duration, seed=4)
@@ -640,6 +703,19 @@ def run_serve(args, Z, ctx):
         cpu = {"value": elems / cs / 1e6, "unit": "Melem/s", "cores": 1, "kind": "port",
                "sample": f"the first {done} requests of the schedule, C oracle LDE, 1 thread, {cs:.1f} s "
                          f"({done / cs:.2f} req/s)"}
+
+        def one(i):
+            _, k, b = arrivals[i]
+            coeffs = O.fill_counter(O.FIELD_FR, 4, i, 1 << k)
+            st, _ = O.lde(O.FIELD_FR, coeffs, k, b, O.from_int(O.FIELD_FR, 5))
+            assert st == O.OK
+            return 1 << (k + b)
+
+        cores = cpu_cores()
+        done, elems, cs = cpu_pool(one, range(len(arrivals)), cores)
+        cpu_all = {"value": elems / cs / 1e6, "unit": "Melem/s", "cores": cores, "kind": "port",
+                   "sample": f"the first {done} requests of the schedule, one per thread, C oracle LDE, "
+                             f"{cores} threads, {cs:.1f} s ({done / cs:.2f} req/s)"}
 
     def summary(r, rate):
         return {"rate_per_s": rate, "requests": r["requests"], "melem_s": r["melem_s"], "req_s": r["req_s"],
@@ -661,7 +737,7 @@ def run_serve(args, Z, ctx):
     line["warmup"] = 3
     line.update({"roofline": roofline, "cpu_baseline": cpu,
                  "extras": {"p50_ms": m["p50_ms"], "p99_ms": m["p99_ms"], "saturated": summary(m, args.rate),
-                            "sub_saturation": summary(sub, sub_rate)}})
+                            "sub_saturation": summary(sub, sub_rate), "cpu_all_cores": cpu_all}})
     return line
 
 
