@@ -96,6 +96,27 @@ def test_callback_transcript_and_statement(Z, ctx, field):
     assert t.challenge_bytes(16) == t2.challenge_bytes(16)
 
 
+@pytest.mark.parametrize("field", ALL, ids=IDS)
+def test_library_transcript_rounds_and_state(Z, ctx, field):
+    """With the library's own transcript the rounds after the first draw
+    their challenges on the device: the proof equals the oracle's, and the
+    transcript is left where the oracle's prover leaves its own (statement
+    bytes absorbed before prove; the next squeezes agree)."""
+    cfg = (16, 12, 6, 2, 8)  # 10 rounds
+    ev = honest_evals(field, 16, 12, 17)
+    want = oracle_proof(field, b"lib", ev, cfg, statement=b"statement bytes")
+    t = Z.Transcript(b"lib")
+    t.absorb_bytes(b"statement bytes")
+    d = dev(ev)
+    got = ctx.fri_prove(field, t, d.data_ptr(), 1 << 16, *cfg, stream=stream())
+    for k in ("roots", "residual", "pair_index", "values", "paths"):
+        assert got[k] == want[k], k
+    t2 = O.Transcript(b"lib")
+    t2.absorb_bytes(b"statement bytes")
+    O.fri_prove(field, t2, ev, *cfg)
+    assert t.challenge_bytes(40) == t2.challenge_bytes(40)
+
+
 def test_random_data_is_rejected(Z, ctx):
     F = O.FIELD_GL
     cfg = (10, 7, 8, 5, 20)
