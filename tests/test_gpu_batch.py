@@ -121,6 +121,32 @@ def test_goldilocks_4096x2p12_against_oracle_digest(Z, ctx):
     b.close()
 
 
+def test_large_transforms_batch_matches_oracle_across_shapes(Z, ctx):
+    """Transforms of three passes (2^19, 2^20) and two (2^17), mixed
+    directions: each equals the oracle's, under the default launch plan and
+    again after the autotuner has chosen its shape."""
+    import torch
+    F = Z.FIELD_FR
+    spec = [(19, 0), (20, 1), (17, 2), (19, 2), (20, 0), (17, 1)]
+    shift = O.from_int(F, 5)
+    xs = [Z.fill_uniform(F, 900 + i, 1 << L) for i, (L, d) in enumerate(spec)]
+    ts = [to_dev(x) for x in xs]
+    items = [(d, L, t.data_ptr(), Z.primitive_root(F, L), shift if d == 2 else None)
+             for (L, d), t in zip(spec, ts)]
+    b = Z.Batch(ctx, F, items)
+    expect = [x.copy() for x in xs]
+    for rnd in range(2):
+        if rnd == 1:
+            b.autotune(reps=1)
+            print("tuned shape", b.shape(), flush=True)
+        b.execute(stream=torch.cuda.current_stream().cuda_stream)
+        b.status(stream=torch.cuda.current_stream().cuda_stream)
+        expect = [oracle_apply(F, e, L, d, shift) for e, (L, d) in zip(expect, spec)]
+        for i, ((L, d), t) in enumerate(zip(spec, ts)):
+            assert np.array_equal(to_host(t, expect[i].shape), expect[i]), (rnd, i, L, d)
+    b.close()
+
+
 def test_uneven_512_batch_equals_per_call(Z, W, ctx):
     """The full uneven workload (512 transforms, 2^10..2^20, mixed
     directions): batched results equal one plain device call per transform."""
