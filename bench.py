@@ -759,7 +759,8 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              "(transforms shard with no communication, see DESIGN.md)", file=sys.stderr)
+              "(batched transforms and requests shard across GPUs with no communication; one large "
+              "transform needs a single all-to-all between its passes)", file=sys.stderr)
         return 2
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import zntt_amd as Z

@@ -40,7 +40,7 @@
  *   - Device-only conditions (HIP errors, device memory, the context's HBM
  *     budget) are reported as ZNTT_DEVICE_ERROR / ZNTT_OUT_OF_MEMORY; a Zig
  *     shim that must keep the reference's exact error sets maps them to a
- *     panic (see INTEGRATION.md).
+ *     panic.
  */
 #ifndef ZNTT_GPU_H
 #define ZNTT_GPU_H
