@@ -36,7 +36,7 @@ __device__ __forceinline__ void round_dyn(typename F::E (&x)[8], const PassArgs 
   // written to LDS at the bit-reversed slot; the round then reads LDS.
   const bool staged = first_round && a.first && c.hi_bits == 0;
   if (staged) {
-    stage_whole_tile<F>(a, lds, c);
+    stage_whole_tile<F, 8, false>(a, lds, c);
     __syncthreads();
   }
   if (active) {

@@ -10,6 +10,25 @@
 
 namespace zntt {
 
+// Transform buffers, tables and scratch are always global memory.  Accessed
+// through address-space-1 pointers they compile to global_* instructions; a
+// pointer read from a descriptor in memory would otherwise give flat_* ones,
+// which also count against the LDS wait counter, so every LDS wait after them
+// would wait for HBM as well.
+#define ZNTT_GLOBAL __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ const ZNTT_GLOBAL T *gld(const void *p) {
+  return (const ZNTT_GLOBAL T *)p;
+}
+template <class T>
+__device__ __forceinline__ ZNTT_GLOBAL T *gst(void *p) {
+  return (ZNTT_GLOBAL T *)p;
+}
+// Plain vector types for wide accesses through those pointers (HIP's uint4
+// class has no copy from another address space).
+typedef uint32_t zu32x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t zu32x4 __attribute__((ext_vector_type(4)));
+
 // ---------------------------------------------------------------------------
 // Field traits for the generic kernels.
 // ---------------------------------------------------------------------------
@@ -18,8 +37,8 @@ struct FrT {
   static constexpr int W = 9;   // dev words
   static constexpr int UW = 8;  // user words
   __device__ static E load_user(const uint32_t *base, size_t i) {
-    const uint4 *p = reinterpret_cast<const uint4 *>(base + i * 8);
-    const uint4 a = p[0], b = p[1];
+    const ZNTT_GLOBAL zu32x4 *p = gld<zu32x4>(base + i * 8);
+    const zu32x4 a = p[0], b = p[1];
     const uint32_t w[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
     return fr_unpack8(w);
   }
@@ -27,19 +46,19 @@ struct FrT {
     fr_canon(x);
     uint32_t w[8];
     fr_pack8(x, w);
-    uint4 *p = reinterpret_cast<uint4 *>(base + i * 8);
-    p[0] = make_uint4(w[0], w[1], w[2], w[3]);
-    p[1] = make_uint4(w[4], w[5], w[6], w[7]);
+    ZNTT_GLOBAL zu32x4 *p = gst<zu32x4>(base + i * 8);
+    p[0] = zu32x4{w[0], w[1], w[2], w[3]};
+    p[1] = zu32x4{w[4], w[5], w[6], w[7]};
   }
   __device__ static E load_dev(const uint32_t *base, size_t i) {
     E x;
-    const uint32_t *p = base + i * 9;
+    const ZNTT_GLOBAL uint32_t *p = gld<uint32_t>(base + i * 9);
 #pragma unroll
     for (int k = 0; k < 9; k++) x.v[k] = p[k];
     return x;
   }
   __device__ static void store_dev(uint32_t *base, size_t i, const E &x) {
-    uint32_t *p = base + i * 9;
+    ZNTT_GLOBAL uint32_t *p = gst<uint32_t>(base + i * 9);
 #pragma unroll
     for (int k = 0; k < 9; k++) p[k] = x.v[k];
   }
@@ -112,16 +131,16 @@ struct GlT {
   static constexpr int W = 2;
   static constexpr int UW = 2;
   __device__ static E load_user(const uint32_t *base, size_t i) {
-    return reinterpret_cast<const uint64_t *>(base)[i];
+    return gld<uint64_t>(base)[i];
   }
   // Transform values are weakly reduced (any u64); the caller's buffer gets
   // the canonical value.
   __device__ static void store_user(uint32_t *base, size_t i, E x) {
-    reinterpret_cast<uint64_t *>(base)[i] = gl_canon(x);
+    gst<uint64_t>(base)[i] = gl_canon(x);
   }
   __device__ static E load_dev(const uint32_t *base, size_t i) { return load_user(base, i); }
   __device__ static void store_dev(uint32_t *base, size_t i, const E &x) {
-    reinterpret_cast<uint64_t *>(base)[i] = x;
+    gst<uint64_t>(base)[i] = x;
   }
   __device__ static E lds_get(const uint32_t *lds, int plane, int i) {
     return (uint64_t)lds[i] | ((uint64_t)lds[plane + i] << 32);
@@ -168,12 +187,10 @@ struct F31T {
   static constexpr int W = 1;
   static constexpr int UW = 2;
   static constexpr uint32_t kP = P;
-  __device__ static E load_user(const uint32_t *base, size_t i) { return base[2 * i]; }
-  __device__ static void store_user(uint32_t *base, size_t i, E x) {
-    reinterpret_cast<uint2 *>(base)[i] = make_uint2(x, 0u);
-  }
-  __device__ static E load_dev(const uint32_t *base, size_t i) { return base[i]; }
-  __device__ static void store_dev(uint32_t *base, size_t i, const E &x) { base[i] = x; }
+  __device__ static E load_user(const uint32_t *base, size_t i) { return gld<uint32_t>(base)[2 * i]; }
+  __device__ static void store_user(uint32_t *base, size_t i, E x) { gst<zu32x2>(base)[i] = zu32x2{x, 0u}; }
+  __device__ static E load_dev(const uint32_t *base, size_t i) { return gld<uint32_t>(base)[i]; }
+  __device__ static void store_dev(uint32_t *base, size_t i, const E &x) { gst<uint32_t>(base)[i] = x; }
   __device__ static E lds_get(const uint32_t *lds, int, int i) { return lds[i]; }
   __device__ static void lds_put(uint32_t *lds, int, int i, const E &x) { lds[i] = x; }
   __device__ static E from_words(const uint32_t *w) { return w[0]; }

@@ -137,9 +137,30 @@ __device__ __forceinline__ void sfor(Fn &&f) {
 // fits one tile): element e of the transform (natural order, coalesced) goes
 // to LDS slot brev(e), multiplied by coset[e] when the pass has
 // a coset, zero at and past src_len.  Uses every thread of the workgroup.
-template <class F>
+// UNROLL: when the workgroup holds exactly N elements a thread and the
+// source covers the tile, every thread issues its N loads before the first
+// LDS write, so the loads of a tile are in flight together rather than one
+// after another.  It costs registers, so kernels whose occupancy depends on a
+// few VGPRs keep the loop.
+template <class F, int N, bool UNROLL>
 __device__ __forceinline__ void stage_whole_tile(const PassArgs &a, uint32_t *lds, const TileCtx &c) {
+  static_assert(N % 2 == 0, "pairs of coset products");
   const int n = c.tile;
+  if (UNROLL && (int)blockDim.x * N == n && a.src_len >= (uint64_t)n) {
+    typename F::E v[N];
+    sfor<N>([&](auto i) { v[i] = F::load_user(a.src, threadIdx.x + i * blockDim.x); });
+    if (a.coset) {
+      sfor<N / 2>([&](auto h) {
+        constexpr int e = 2 * decltype(h)::value;
+        F::mul2(v[e], F::load_dev(a.coset, threadIdx.x + e * blockDim.x), v[e + 1],
+                F::load_dev(a.coset, threadIdx.x + (e + 1) * blockDim.x));
+      });
+    }
+    sfor<N>([&](auto i) {
+      F::lds_put(lds, c.tile, lds_swz(brev_bits(threadIdx.x + i * blockDim.x, c.K)), v[i]);
+    });
+    return;
+  }
   for (int e = threadIdx.x; e < n; e += blockDim.x) {
     typename F::E v = F::zero();
     if ((uint64_t)e < a.src_len) {
@@ -176,7 +197,9 @@ struct StageMap {
 
 // TWL: the pass's twiddle table is resident in LDS after the tile's planes
 // (a whole transform: 2^K entries, plane stride 2^K), not read from HBM.
-template <class F, int R, int l0, bool first_round, bool last_round, bool ODD, bool TWL = false, int N = 8>
+// WHOLE: the round runs in the whole-transform batch kernel.
+template <class F, int R, int l0, bool first_round, bool last_round, bool ODD, bool TWL = false, int N = 8,
+          bool WHOLE = false>
 __device__ __forceinline__ void pass_round(typename F::E (&x)[N], const PassArgs &a, uint32_t *lds,
                                            const TileCtx &c) {
   using E = typename F::E;
@@ -212,7 +235,7 @@ __device__ __forceinline__ void pass_round(typename F::E (&x)[N], const PassArgs
   if constexpr (first_round) {
     staged = a.first && c.hi_bits == 0;
     if (staged) {
-      stage_whole_tile<F>(a, lds, c);
+      stage_whole_tile<F, N, WHOLE || !std::is_same<F, FrT>::value>(a, lds, c);
       __syncthreads();
     }
   }
@@ -318,7 +341,7 @@ __device__ __forceinline__ void pass_round(typename F::E (&x)[N], const PassArgs
 // Rounds of a K-stage pass over N = 2^LN elements a thread: the first takes
 // K - LN*(ceil(K/LN) - 1) stages, the rest LN each; unrolled at compile time
 // so `x` stays in registers.
-template <class F, int K, int RI, bool ODD, bool TWL = false, int N = 8>
+template <class F, int K, int RI, bool ODD, bool TWL = false, int N = 8, bool WHOLE = false>
 __device__ __forceinline__ void pass_rounds(typename F::E (&x)[N], const PassArgs &a, uint32_t *lds,
                                             const TileCtx &c) {
   constexpr int LN = N == 8 ? 3 : 2;
@@ -326,10 +349,10 @@ __device__ __forceinline__ void pass_rounds(typename F::E (&x)[N], const PassArg
   constexpr int R0 = K - LN * (NR - 1);
   constexpr int R = RI == 0 ? R0 : LN;
   constexpr int L0 = RI == 0 ? 0 : R0 + LN * (RI - 1);
-  pass_round<F, R, L0, RI == 0, RI == NR - 1, ODD, TWL, N>(x, a, lds, c);
+  pass_round<F, R, L0, RI == 0, RI == NR - 1, ODD, TWL, N, WHOLE>(x, a, lds, c);
   if constexpr (RI + 1 < NR) {
     __syncthreads();
-    pass_rounds<F, K, RI + 1, ODD, TWL, N>(x, a, lds, c);
+    pass_rounds<F, K, RI + 1, ODD, TWL, N, WHOLE>(x, a, lds, c);
   }
 }
 
@@ -369,6 +392,14 @@ __device__ __forceinline__ void retire_workgroup(uint32_t *ctr) {
   }
 }
 
+// Waves a SIMD that the whole-transform kernel asks the register allocator
+// for: two 512-thread workgroups a CU (the LDS allows two for every field)
+// need four, i.e. at most 128 VGPRs.  BN254 Fr's values do not fit that.
+template <class F>
+constexpr int whole_waves_per_simd() {
+  return std::is_same<F, FrT>::value ? 1 : 4;
+}
+
 // Persistent kernel over a batch of transforms that each fit one tile: item
 // i is transform ent[i].xf, run as a single pass of K = L stages with the
 // register rounds above (coalesced staging, compile-time indexing).  Items
@@ -378,8 +409,8 @@ __device__ __forceinline__ void retire_workgroup(uint32_t *ctr) {
 // transform of the launch uses the table `tw_all` (2^K entries), which each
 // workgroup copies into LDS once, behind the tile's planes.
 template <class F, int K, bool TWL>
-__global__ __launch_bounds__(512) void whole_batch_kernel(const BatchXform *xf, const BatchEntry *ent, uint32_t n,
-                                                          uint32_t *next, const uint32_t *tw_all) {
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(whole_waves_per_simd<F>()))) void whole_batch_kernel(const BatchXform *__restrict__ xf, const BatchEntry *__restrict__ ent, uint32_t n,
+                                                          uint32_t *next, const uint32_t *__restrict__ tw_all) {
   using E = typename F::E;
   extern __shared__ uint32_t lds[];
   __shared__ uint32_t s_item;
@@ -389,7 +420,10 @@ __global__ __launch_bounds__(512) void whole_batch_kernel(const BatchXform *xf, 
   }
   if (threadIdx.x == 0) s_item = atomicAdd(next, 1u);
   __syncthreads();
-  uint32_t it = s_item;
+  // The item index is the same in every lane: read it as a scalar, so the
+  // descriptor loads below are scalar loads and the buffer pointers stay in
+  // SGPRs.
+  uint32_t it = __builtin_amdgcn_readfirstlane(s_item);
   while (it < n) {
     uint32_t claimed = 0;
     if (threadIdx.x == 0) claimed = atomicAdd(next, 1u);
@@ -420,11 +454,11 @@ __global__ __launch_bounds__(512) void whole_batch_kernel(const BatchXform *xf, 
     c.hi = c.lo0 = c.c0 = 0;
     constexpr int N = round_elems<F>();
     E x[N];
-    pass_rounds<F, K, 0, true, TWL, N>(x, a, lds, c);
+    pass_rounds<F, K, 0, true, TWL, N, true>(x, a, lds, c);
     __syncthreads();  // LDS is reused by the next item
     if (threadIdx.x == 0) s_item = claimed;
     __syncthreads();
-    it = s_item;
+    it = __builtin_amdgcn_readfirstlane(s_item);
   }
   if (threadIdx.x == 0) retire_workgroup(next);
 }
