@@ -114,6 +114,25 @@ __global__ __launch_bounds__(256) void ntt_pass_kernel(const PassArgs a) {
 // sends those runs to distinct LDS banks.  A bijection on any tile >= 256.
 __device__ __forceinline__ int lds_swz(int i) { return i ^ (((i >> 6) & 3) << 3); }
 
+// The whole-transform kernel's tile (one transform, G = 1) uses another
+// swizzle: bit 5 + j of a tile index, j < 7, XORs column kWholeSwzCol[j] into
+// bits 0..5.  Bits 6 and up are kept, bit 5 changes by a function of them
+// and bits 0..4 by a function of bits 5 and up, so the map is a bijection on
+// every 2^K tile.  The columns were searched so that, for K = 9..12 and four
+// or eight elements a thread, the 64 lanes of every staging write and of
+// every round's reads and write-backs hit 64 distinct LDS banks (under 32
+// banks, each half-wave hits 32, except 2^10 tiles at 1.2-way on average);
+// the shift-and-mask swizzle above reaches 4-5-way conflicts on these
+// patterns.  The map is linear over GF(2): for indices with disjoint bits,
+// whole_swz(a | b) = whole_swz(a) ^ whole_swz(b), so a set's positions are
+// its base's image XOR compile-time constants.
+constexpr int kWholeSwzCol[7] = {31, 42, 49, 35, 19, 4, 28};
+__host__ __device__ constexpr int whole_swz(int i) {
+  int m = 0;
+  for (int j = 0; j < 7; j++) m ^= (-((i >> (5 + j)) & 1)) & kWholeSwzCol[j];
+  return i ^ m;
+}
+
 struct TileCtx {
   int K, lg, G, tile, lo_bits, hi_bits;
   size_t hi, lo0, c0;
@@ -141,8 +160,9 @@ __device__ __forceinline__ void sfor(Fn &&f) {
 // source covers the tile, every thread issues its N loads before the first
 // LDS write, so the loads of a tile are in flight together rather than one
 // after another.  It costs registers, so kernels whose occupancy depends on a
-// few VGPRs keep the loop.
-template <class F, int N, bool UNROLL>
+// few VGPRs keep the loop.  WHOLE: the tile uses whole_swz (the
+// whole-transform kernel, where blockDim.x = 2^K / N).
+template <class F, int N, bool UNROLL, bool WHOLE = false>
 __device__ __forceinline__ void stage_whole_tile(const PassArgs &a, uint32_t *lds, const TileCtx &c) {
   static_assert(N % 2 == 0, "pairs of coset products");
   const int n = c.tile;
@@ -156,9 +176,17 @@ __device__ __forceinline__ void stage_whole_tile(const PassArgs &a, uint32_t *ld
                 F::load_dev(a.coset, threadIdx.x + (e + 1) * blockDim.x));
       });
     }
-    sfor<N>([&](auto i) {
-      F::lds_put(lds, c.tile, lds_swz(brev_bits(threadIdx.x + i * blockDim.x, c.K)), v[i]);
-    });
+    if constexpr (WHOLE) {
+      // brev(t + i * T) = brev(t) | brev(i * T) for t < T = n / N.
+      const int ts = whole_swz(brev_bits(threadIdx.x, c.K));
+      sfor<N>([&](auto i) {
+        F::lds_put(lds, c.tile, ts ^ whole_swz(brev_bits(decltype(i)::value * (n / N), c.K)), v[i]);
+      });
+    } else {
+      sfor<N>([&](auto i) {
+        F::lds_put(lds, c.tile, lds_swz(brev_bits(threadIdx.x + i * blockDim.x, c.K)), v[i]);
+      });
+    }
     return;
   }
   for (int e = threadIdx.x; e < n; e += blockDim.x) {
@@ -167,7 +195,8 @@ __device__ __forceinline__ void stage_whole_tile(const PassArgs &a, uint32_t *ld
       v = F::load_user(a.src, e);
       if (a.coset) v = F::mul(v, F::load_dev(a.coset, e));
     }
-    F::lds_put(lds, c.tile, lds_swz(brev_bits(e, c.K)), v);
+    const int r = brev_bits(e, c.K);
+    F::lds_put(lds, c.tile, WHOLE ? whole_swz(r) : lds_swz(r), v);
   }
 }
 
@@ -211,6 +240,15 @@ __device__ __forceinline__ void pass_round(typename F::E (&x)[N], const PassArgs
     const int rest = slot >> c.lg;
     base[i] = (rest & ((1 << l0) - 1)) | ((rest >> l0) << (l0 + R));
   });
+  // LDS slot of element k of set i.
+  int sb[NSETS];
+  if constexpr (WHOLE) sfor<NSETS>([&](auto i) { sb[i] = whole_swz(base[i]); });
+  auto slot_of = [&](auto i, auto k) {
+    if constexpr (WHOLE)
+      return sb[i] ^ whole_swz(decltype(k)::value << l0);
+    else
+      return lds_swz((base[i] | (decltype(k)::value << l0)) * c.G + g[i]);
+  };
   // Twiddles: stage l0+t couples (k, k + 2^t) with index jl = mid mod 2^(l0+t),
   // so a set needs 2^t of them at stage t; issue all loads before computing.
   E tw[NSETS * (SET - 1)];
@@ -235,7 +273,7 @@ __device__ __forceinline__ void pass_round(typename F::E (&x)[N], const PassArgs
   if constexpr (first_round) {
     staged = a.first && c.hi_bits == 0;
     if (staged) {
-      stage_whole_tile<F, N, WHOLE || !std::is_same<F, FrT>::value>(a, lds, c);
+      stage_whole_tile<F, N, WHOLE || !std::is_same<F, FrT>::value, WHOLE>(a, lds, c);
       __syncthreads();
     }
   }
@@ -244,7 +282,7 @@ __device__ __forceinline__ void pass_round(typename F::E (&x)[N], const PassArgs
       const int mid = base[i] | (k << l0);
       if constexpr (first_round) {
         if (staged) {
-          x[i * SET + k] = F::lds_get(lds, c.tile, lds_swz(mid * c.G + g[i]));
+          x[i * SET + k] = F::lds_get(lds, c.tile, slot_of(i, k));
         } else if (a.first) {
           const size_t src = ((size_t)brev_bits(mid, c.K) << c.hi_bits) + c.c0 + g[i];
           x[i * SET + k] = F::load_user(a.src, src);
@@ -253,7 +291,7 @@ __device__ __forceinline__ void pass_round(typename F::E (&x)[N], const PassArgs
           x[i * SET + k] = F::load_dev(a.src, pos);
         }
       } else {
-        x[i * SET + k] = F::lds_get(lds, c.tile, lds_swz(mid * c.G + g[i]));
+        x[i * SET + k] = F::lds_get(lds, c.tile, slot_of(i, k));
       }
     });
   });
@@ -332,7 +370,7 @@ __device__ __forceinline__ void pass_round(typename F::E (&x)[N], const PassArgs
         else
           F::store_dev(a.dst, pos, x[i * SET + k]);
       } else {
-        F::lds_put(lds, c.tile, lds_swz(mid * c.G + g[i]), x[i * SET + k]);
+        F::lds_put(lds, c.tile, slot_of(i, k), x[i * SET + k]);
       }
     });
   });
