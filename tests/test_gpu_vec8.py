@@ -33,7 +33,7 @@ def rand(n, seed):
     return np.random.default_rng(seed).integers(0, P, size=n, dtype=np.uint64)
 
 
-@pytest.mark.parametrize("log_n", [0, 1, 2, 5, 11, 12, 13, 16, 20])
+@pytest.mark.parametrize("log_n", [0, 1, 2, 5, 6, 7, 8, 9, 10, 11, 12, 13, 16, 20])
 @pytest.mark.parametrize("inverse", [False, True], ids=["fwd", "inv"])
 def test_host_matches_oracle(Z, ctx, log_n, inverse):
     n = 1 << log_n

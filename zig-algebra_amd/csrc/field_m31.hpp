@@ -14,6 +14,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "field_traits.hpp"
+
 namespace zntt {
 
 struct M31Arith {
@@ -46,12 +48,10 @@ struct M31T {
   static constexpr int W = 1;
   static constexpr int UW = 2;
   static constexpr uint32_t kP = A::P;
-  __device__ static E load_user(const uint32_t *base, size_t i) { return base[2 * i]; }
-  __device__ static void store_user(uint32_t *base, size_t i, E x) {
-    reinterpret_cast<uint2 *>(base)[i] = make_uint2(x, 0u);
-  }
-  __device__ static E load_dev(const uint32_t *base, size_t i) { return base[i]; }
-  __device__ static void store_dev(uint32_t *base, size_t i, const E &x) { base[i] = x; }
+  __device__ static E load_user(const uint32_t *base, size_t i) { return gld<uint32_t>(base)[2 * i]; }
+  __device__ static void store_user(uint32_t *base, size_t i, E x) { gst<zu32x2>(base)[i] = zu32x2{x, 0u}; }
+  __device__ static E load_dev(const uint32_t *base, size_t i) { return gld<uint32_t>(base)[i]; }
+  __device__ static void store_dev(uint32_t *base, size_t i, const E &x) { gst<uint32_t>(base)[i] = x; }
   __device__ static E lds_get(const uint32_t *lds, int, int i) { return lds[i]; }
   __device__ static void lds_put(uint32_t *lds, int, int i, const E &x) { lds[i] = x; }
   __device__ static E from_words(const uint32_t *w) { return w[0]; }

@@ -120,6 +120,12 @@ hipError_t launch_transform_m31(int L, uint32_t *data, uint32_t *scratch, const 
                                 const uint32_t *scale_dev, hipStream_t stream);
 hipError_t launch_m31_table(int L, uint32_t *flat, const uint32_t *pows, hipStream_t s);
 hipError_t launch_m31_lanes(bool gather, const void *src, void *dst, uint64_t n, hipStream_t s);
+// All eight interleaved lanes of 2^L u64 values each, transformed in place
+// without gathering them (data[i * 8 + lane]; scratch: 8 * 2^L u32).  For
+// 6 <= L <= 28; returns hipErrorInvalidValue outside that range.
+constexpr int kM31LanesMinLog = 6;
+hipError_t launch_transform_m31_lanes(int L, uint32_t *data, uint32_t *scratch, const uint32_t *tw,
+                                      const uint32_t *scale_dev, hipStream_t stream);
 // Elementwise device field op (0 add, 1 sub, 2 mul) on user-layout arrays.
 hipError_t launch_field_op(int field, int op, size_t n, const uint32_t *a, const uint32_t *b, uint32_t *out,
                            hipStream_t stream);

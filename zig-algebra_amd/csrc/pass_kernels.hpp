@@ -227,19 +227,29 @@ struct StageMap {
 // TWL: the pass's twiddle table is resident in LDS after the tile's planes
 // (a whole transform: 2^K entries, plane stride 2^K), not read from HBM.
 // WHOLE: the round runs in the whole-transform batch kernel.
+// LB: lane bits.  The buffers hold 2^LB transforms of the same size
+// interleaved, element e of transform `lane` at index e * 2^LB + lane, and
+// the low LB bits of a tile column select the lane: the tile holds the same
+// 2^lg / 2^LB columns of every lane, so consecutive threads touch consecutive
+// words.  Positions and twiddle indices use the column without the lane bits
+// (c.c0, c.lo0 count such columns); LDS slots use the full column.
 template <class F, int R, int l0, bool first_round, bool last_round, bool ODD, bool TWL = false, int N = 8,
-          bool WHOLE = false>
+          bool WHOLE = false, int LB = 0>
 __device__ __forceinline__ void pass_round(typename F::E (&x)[N], const PassArgs &a, uint32_t *lds,
                                            const TileCtx &c) {
   using E = typename F::E;
   constexpr int SET = 1 << R, NSETS = N >> R;
-  int g[NSETS], base[NSETS];
+  int g[NSETS], base[NSETS], gr[NSETS], ln[NSETS];
   sfor<NSETS>([&](auto i) {
     const int slot = threadIdx.x + i * blockDim.x;
     g[i] = slot & (c.G - 1);
+    gr[i] = g[i] >> LB;
+    ln[i] = g[i] & ((1 << LB) - 1);
     const int rest = slot >> c.lg;
     base[i] = (rest & ((1 << l0) - 1)) | ((rest >> l0) << (l0 + R));
   });
+  // Buffer index of transform element `e` of set i's lane.
+  auto at = [&](auto i, size_t e) { return (e << LB) + ln[i]; };
   // LDS slot of element k of set i.
   int sb[NSETS];
   if constexpr (WHOLE) sfor<NSETS>([&](auto i) { sb[i] = whole_swz(base[i]); });
@@ -256,7 +266,7 @@ __device__ __forceinline__ void pass_round(typename F::E (&x)[N], const PassArgs
     const int s = a.s0 + l0 + t;
     const size_t off = ((size_t)1 << (s - 1)) - 1;
     sfor<NSETS>([&](auto i) {
-      const size_t lo = a.first ? 0 : c.lo0 + g[i];
+      const size_t lo = a.first ? 0 : c.lo0 + gr[i];
       const int jbase = base[i] & ((1 << l0) - 1);
       sfor<(1 << t)>([&](auto kk) {
         const size_t ti = off + ((size_t)(jbase | (kk << l0)) << c.lo_bits) + lo;
@@ -271,7 +281,7 @@ __device__ __forceinline__ void pass_round(typename F::E (&x)[N], const PassArgs
   // first (coalesced reads instead of a bit-reversed gather).
   bool staged = false;
   if constexpr (first_round) {
-    staged = a.first && c.hi_bits == 0;
+    staged = LB == 0 && a.first && c.hi_bits == 0;
     if (staged) {
       stage_whole_tile<F, N, WHOLE || !std::is_same<F, FrT>::value, WHOLE>(a, lds, c);
       __syncthreads();
@@ -284,11 +294,11 @@ __device__ __forceinline__ void pass_round(typename F::E (&x)[N], const PassArgs
         if (staged) {
           x[i * SET + k] = F::lds_get(lds, c.tile, slot_of(i, k));
         } else if (a.first) {
-          const size_t src = ((size_t)brev_bits(mid, c.K) << c.hi_bits) + c.c0 + g[i];
-          x[i * SET + k] = F::load_user(a.src, src);
+          const size_t src = ((size_t)brev_bits(mid, c.K) << c.hi_bits) + c.c0 + gr[i];
+          x[i * SET + k] = F::load_user(a.src, at(i, src));
         } else {
-          const size_t pos = (c.hi << (c.lo_bits + c.K)) + ((size_t)mid << c.lo_bits) + c.lo0 + g[i];
-          x[i * SET + k] = F::load_dev(a.src, pos);
+          const size_t pos = (c.hi << (c.lo_bits + c.K)) + ((size_t)mid << c.lo_bits) + c.lo0 + gr[i];
+          x[i * SET + k] = F::load_dev(a.src, at(i, pos));
         }
       } else {
         x[i * SET + k] = F::lds_get(lds, c.tile, slot_of(i, k));
@@ -301,7 +311,7 @@ __device__ __forceinline__ void pass_round(typename F::E (&x)[N], const PassArgs
       auto cos = [&](int e) {
         const int i = e / SET, k = e % SET;
         const int mid = base[i] | (k << l0);
-        return F::load_dev(a.coset, ((size_t)brev_bits(mid, c.K) << c.hi_bits) + c.c0 + g[i]);
+        return F::load_dev(a.coset, ((size_t)brev_bits(mid, c.K) << c.hi_bits) + c.c0 + gr[i]);
       };
       sfor<N / 2>([&](auto h) {
         constexpr int e = 2 * decltype(h)::value;
@@ -362,13 +372,13 @@ __device__ __forceinline__ void pass_round(typename F::E (&x)[N], const PassArgs
       if constexpr (last_round) {
         size_t pos;
         if (a.first)
-          pos = ((size_t)brev_bits((uint32_t)(c.c0 + g[i]), c.hi_bits) << c.K) + mid;
+          pos = ((size_t)brev_bits((uint32_t)(c.c0 + gr[i]), c.hi_bits) << c.K) + mid;
         else
-          pos = (c.hi << (c.lo_bits + c.K)) + ((size_t)mid << c.lo_bits) + c.lo0 + g[i];
+          pos = (c.hi << (c.lo_bits + c.K)) + ((size_t)mid << c.lo_bits) + c.lo0 + gr[i];
         if (a.last)
-          F::store_user(a.dst, pos, x[i * SET + k]);
+          F::store_user(a.dst, at(i, pos), x[i * SET + k]);
         else
-          F::store_dev(a.dst, pos, x[i * SET + k]);
+          F::store_dev(a.dst, at(i, pos), x[i * SET + k]);
       } else {
         F::lds_put(lds, c.tile, slot_of(i, k), x[i * SET + k]);
       }
@@ -379,7 +389,7 @@ __device__ __forceinline__ void pass_round(typename F::E (&x)[N], const PassArgs
 // Rounds of a K-stage pass over N = 2^LN elements a thread: the first takes
 // K - LN*(ceil(K/LN) - 1) stages, the rest LN each; unrolled at compile time
 // so `x` stays in registers.
-template <class F, int K, int RI, bool ODD, bool TWL = false, int N = 8, bool WHOLE = false>
+template <class F, int K, int RI, bool ODD, bool TWL = false, int N = 8, bool WHOLE = false, int LB = 0>
 __device__ __forceinline__ void pass_rounds(typename F::E (&x)[N], const PassArgs &a, uint32_t *lds,
                                             const TileCtx &c) {
   constexpr int LN = N == 8 ? 3 : 2;
@@ -387,14 +397,16 @@ __device__ __forceinline__ void pass_rounds(typename F::E (&x)[N], const PassArg
   constexpr int R0 = K - LN * (NR - 1);
   constexpr int R = RI == 0 ? R0 : LN;
   constexpr int L0 = RI == 0 ? 0 : R0 + LN * (RI - 1);
-  pass_round<F, R, L0, RI == 0, RI == NR - 1, ODD, TWL, N, WHOLE>(x, a, lds, c);
+  pass_round<F, R, L0, RI == 0, RI == NR - 1, ODD, TWL, N, WHOLE, LB>(x, a, lds, c);
   if constexpr (RI + 1 < NR) {
     __syncthreads();
-    pass_rounds<F, K, RI + 1, ODD, TWL, N, WHOLE>(x, a, lds, c);
+    pass_rounds<F, K, RI + 1, ODD, TWL, N, WHOLE, LB>(x, a, lds, c);
   }
 }
 
-template <class F, int K, bool ODD, int N = 8>
+// LB > 0: 2^LB interleaved transforms (see pass_round); a.log_g counts the
+// lane bits, a.L and a.s0 are those of one transform.
+template <class F, int K, bool ODD, int N = 8, int LB = 0>
 __global__ __launch_bounds__(512) void ntt_pass_r8_kernel(const PassArgs a) {
   using E = typename F::E;
   extern __shared__ uint32_t lds[];
@@ -405,17 +417,18 @@ __global__ __launch_bounds__(512) void ntt_pass_r8_kernel(const PassArgs a) {
   c.tile = c.G << c.K;
   c.lo_bits = a.s0 - 1;
   c.hi_bits = a.L - c.lo_bits - c.K;
+  const int lgr = c.lg - LB;  // columns of one transform in the tile
   const size_t w = blockIdx.x;
   c.hi = c.lo0 = c.c0 = 0;
   if (a.first) {
-    c.c0 = w << c.lg;
+    c.c0 = w << lgr;
   } else {
-    const int lbb = c.lo_bits - c.lg;
+    const int lbb = c.lo_bits - lgr;
     c.hi = w >> lbb;
-    c.lo0 = (w & (((size_t)1 << lbb) - 1)) << c.lg;
+    c.lo0 = (w & (((size_t)1 << lbb) - 1)) << lgr;
   }
   E x[N];
-  pass_rounds<F, K, 0, ODD, false, N>(x, a, lds, c);
+  pass_rounds<F, K, 0, ODD, false, N, false, LB>(x, a, lds, c);
 }
 
 // Persistent launches claim items from ctr[0] and count finished workgroups

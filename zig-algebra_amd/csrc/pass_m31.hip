@@ -62,6 +62,55 @@ hipError_t launch_transform_m31(int L, uint32_t *data, uint32_t *scratch, const 
   return run_passes_t<M31T>(kM31, L, data, scratch, tw, nullptr, scale_dev, true, stream);
 }
 
+// The eight lanes in place: every pass is one launch of the register-round
+// pass kernel with three lane bits, so a tile holds the same 2^lg columns of
+// all eight transforms (2^K * 2^lg * 8 <= 2^12 words) and reads and writes
+// data[i * 8 + lane] with consecutive threads on consecutive words.  The
+// intermediate layout in `scratch` is interleaved the same way (u32).
+hipError_t launch_transform_m31_lanes(int L, uint32_t *data, uint32_t *scratch, const uint32_t *tw,
+                                      const uint32_t *scale_dev, hipStream_t stream) {
+  constexpr int LB = 3;
+  if (L < kM31LanesMinLog || L > 28) return hipErrorInvalidValue;
+  const PassPlan plan = plan_passes_shape(L, 12 - LB, 12 - LB);
+  int s0 = 1;
+  for (int i = 0; i < plan.passes; i++) {
+    const int K = plan.K[i], lg = plan.log_g[i];
+    const size_t tile = (size_t)1 << (K + lg + LB);
+    // The kernel's shape: at least 512 words a tile, 8 elements a thread.
+    if (K < 5 || K > 9 || tile < 512 || tile > 4096) return hipErrorInvalidValue;
+    PassArgs a = {};
+    a.first = i == 0;
+    a.last = i == plan.passes - 1;
+    a.src = a.first ? data : scratch;
+    a.dst = a.last ? data : scratch;
+    a.tw = tw;
+    a.coset = nullptr;
+    a.has_scale = (a.last && scale_dev) ? 1 : 0;
+    if (a.has_scale) a.scale[0] = scale_dev[0];
+    a.L = L;
+    a.src_len = (uint64_t)1 << L;
+    a.tw_one = 1;
+    a.s0 = s0;
+    a.K = K;
+    a.log_g = lg + LB;
+    const dim3 grid((unsigned)(((size_t)1 << L) >> (K + lg))), block((unsigned)(tile / 8));
+    const size_t lds = tile * 4;
+    const bool odd = (s0 & 1) != 0;
+    switch (K * 2 + (odd ? 1 : 0)) {
+#define ZNTT_M31L(k)                                                                                            \
+  case 2 * k: hipLaunchKernelGGL((ntt_pass_r8_kernel<M31T, k, false, 8, LB>), grid, block, lds, stream, a); break; \
+  case 2 * k + 1: hipLaunchKernelGGL((ntt_pass_r8_kernel<M31T, k, true, 8, LB>), grid, block, lds, stream, a); break;
+      ZNTT_M31L(5) ZNTT_M31L(6) ZNTT_M31L(7) ZNTT_M31L(8) ZNTT_M31L(9)
+#undef ZNTT_M31L
+      default: return hipErrorInvalidValue;
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    s0 += K;
+  }
+  return hipSuccess;
+}
+
 hipError_t launch_m31_table(int L, uint32_t *flat, const uint32_t *pows, hipStream_t s) {
   const uint64_t n = 1ull << L;
   hipLaunchKernelGGL(m31_table_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, flat, L, pows);

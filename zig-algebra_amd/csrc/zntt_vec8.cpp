@@ -35,9 +35,13 @@ int vec8_device(zntt_ctx *c, bool inverse, void *data, int L, uint64_t root, hip
   // The inverse runs the forward DAG with root^-1, then scales by n^-1
   // (lib.zig:114-129).
   const uint64_t r = inverse ? m31_inv(root) : root;
-  // Context memory: eight lane-major transforms (u64), the pass scratch and
-  // the twiddle table (u32 each), the stage powers.
-  const size_t lanes_b = 8 * n * 8, scratch_b = n * 4, table_b = n * 4, pows_b = 64 * 4;
+  // From 2^6 values a lane the passes work on the interleaved buffer itself
+  // (scratch: 8 * n u32); smaller transforms gather the lanes first.
+  // Context memory: eight lane-major transforms (u64, gathered form only),
+  // the pass scratch and the twiddle table (u32 each), the stage powers.
+  const bool fused = L >= zntt::kM31LanesMinLog;
+  const size_t lanes_b = fused ? 0 : 8 * n * 8, scratch_b = (fused ? 8 : 1) * n * 4, table_b = n * 4,
+               pows_b = 64 * 4;
   int st = ensure(c, &c->aux, &c->aux_bytes, lanes_b + scratch_b + table_b + pows_b);
   if (st) return st;
   if ((st = ensure_pinned(c, pows_b))) return st;
@@ -55,11 +59,16 @@ int vec8_device(zntt_ctx *c, bool inverse, void *data, int L, uint64_t root, hip
   uint32_t scale[9] = {(uint32_t)m31_inv(n % kP)};
   hipError_t e = hipMemcpyAsync(pows_d, pows, (size_t)L * 4, hipMemcpyHostToDevice, s);
   if (e == hipSuccess) e = zntt::launch_m31_table(L, table, pows_d, s);
-  if (e == hipSuccess) e = zntt::launch_m31_lanes(true, data, lanes, n, s);
-  for (int lane = 0; lane < 8 && e == hipSuccess; lane++)
-    e = zntt::launch_transform_m31(L, (uint32_t *)(lanes + (size_t)lane * n), scratch, table,
-                                   inverse ? scale : nullptr, s);
-  if (e == hipSuccess) e = zntt::launch_m31_lanes(false, lanes, data, n, s);
+  if (fused) {
+    if (e == hipSuccess)
+      e = zntt::launch_transform_m31_lanes(L, (uint32_t *)data, scratch, table, inverse ? scale : nullptr, s);
+  } else {
+    if (e == hipSuccess) e = zntt::launch_m31_lanes(true, data, lanes, n, s);
+    for (int lane = 0; lane < 8 && e == hipSuccess; lane++)
+      e = zntt::launch_transform_m31(L, (uint32_t *)(lanes + (size_t)lane * n), scratch, table,
+                                     inverse ? scale : nullptr, s);
+    if (e == hipSuccess) e = zntt::launch_m31_lanes(false, lanes, data, n, s);
+  }
   // The staged powers are read by the copy above; keep them until it is done.
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   return hip_status(e);
