@@ -63,6 +63,12 @@ struct zntt_ctx {
   size_t aux_bytes = 0;
   void *pinned = nullptr;  // page-locked host staging (small read-backs and uploads)
   size_t pinned_bytes = 0;
+  // The eight-lane M31 transform's twiddle table for (m31_root, m31_log_n);
+  // kept so that repeated calls with one root skip the rebuild.
+  void *m31_table = nullptr;
+  size_t m31_table_bytes = 0;
+  uint64_t m31_root = 0;
+  int m31_log_n = -1;
 };
 
 namespace zntt_detail {

@@ -482,8 +482,10 @@ int zntt_ctx_trim(zntt_ctx *c) {
   dev_free(c, c->scratch, c->scratch_bytes);
   dev_free(c, c->staging, c->staging_bytes);
   dev_free(c, c->aux, c->aux_bytes);
-  c->scratch = c->staging = c->aux = nullptr;
-  c->scratch_bytes = c->staging_bytes = c->aux_bytes = 0;
+  dev_free(c, c->m31_table, c->m31_table_bytes);
+  c->scratch = c->staging = c->aux = c->m31_table = nullptr;
+  c->scratch_bytes = c->staging_bytes = c->aux_bytes = c->m31_table_bytes = 0;
+  c->m31_log_n = -1;
   if (c->pinned) hipHostFree(c->pinned);
   c->pinned = nullptr;
   c->pinned_bytes = 0;

@@ -38,27 +38,35 @@ int vec8_device(zntt_ctx *c, bool inverse, void *data, int L, uint64_t root, hip
   // From 2^6 values a lane the passes work on the interleaved buffer itself
   // (scratch: 8 * n u32); smaller transforms gather the lanes first.
   // Context memory: eight lane-major transforms (u64, gathered form only),
-  // the pass scratch and the twiddle table (u32 each), the stage powers.
+  // the pass scratch (u32) and the stage powers; the twiddle table (u32) has
+  // a buffer of its own and is rebuilt only when (root, L) changes.
   const bool fused = L >= zntt::kM31LanesMinLog;
   const size_t lanes_b = fused ? 0 : 8 * n * 8, scratch_b = (fused ? 8 : 1) * n * 4, table_b = n * 4,
                pows_b = 64 * 4;
-  int st = ensure(c, &c->aux, &c->aux_bytes, lanes_b + scratch_b + table_b + pows_b);
+  int st = ensure(c, &c->aux, &c->aux_bytes, lanes_b + scratch_b + pows_b);
   if (st) return st;
-  if ((st = ensure_pinned(c, pows_b))) return st;
   uint8_t *base = (uint8_t *)c->aux;
   uint64_t *lanes = (uint64_t *)base;
   uint32_t *scratch = (uint32_t *)(base + lanes_b);
-  uint32_t *table = (uint32_t *)(base + lanes_b + scratch_b);
-  uint32_t *pows_d = (uint32_t *)(base + lanes_b + scratch_b + table_b);
-  uint32_t *pows = (uint32_t *)c->pinned;
-  uint64_t x = r % kP;
-  for (int k = 0; k < L; k++) {
-    pows[k] = (uint32_t)x;
-    x = m31_mul(x, x);
+  uint32_t *pows_d = (uint32_t *)(base + lanes_b + scratch_b);
+  const uint64_t rk = r % kP;
+  const bool cached = c->m31_table && c->m31_log_n == L && c->m31_root == rk;
+  hipError_t e = hipSuccess;
+  if (!cached) {
+    c->m31_log_n = -1;  // no valid table until this call's build has completed
+    if ((st = ensure(c, &c->m31_table, &c->m31_table_bytes, table_b))) return st;
+    if ((st = ensure_pinned(c, pows_b))) return st;
+    uint32_t *pows = (uint32_t *)c->pinned;
+    uint64_t x = rk;
+    for (int k = 0; k < L; k++) {
+      pows[k] = (uint32_t)x;
+      x = m31_mul(x, x);
+    }
+    e = hipMemcpyAsync(pows_d, pows, (size_t)L * 4, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = zntt::launch_m31_table(L, (uint32_t *)c->m31_table, pows_d, s);
   }
+  const uint32_t *table = (const uint32_t *)c->m31_table;
   uint32_t scale[9] = {(uint32_t)m31_inv(n % kP)};
-  hipError_t e = hipMemcpyAsync(pows_d, pows, (size_t)L * 4, hipMemcpyHostToDevice, s);
-  if (e == hipSuccess) e = zntt::launch_m31_table(L, table, pows_d, s);
   if (fused) {
     if (e == hipSuccess)
       e = zntt::launch_transform_m31_lanes(L, (uint32_t *)data, scratch, table, inverse ? scale : nullptr, s);
@@ -71,6 +79,10 @@ int vec8_device(zntt_ctx *c, bool inverse, void *data, int L, uint64_t root, hip
   }
   // The staged powers are read by the copy above; keep them until it is done.
   if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e == hipSuccess && !cached) {
+    c->m31_root = rk;
+    c->m31_log_n = L;
+  }
   return hip_status(e);
 }
 
