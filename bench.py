@@ -658,6 +658,47 @@ def serve_muls(arrivals):
     return tot
 
 
+def serve_per_call(args, Z, ctx, field, shift, budget_s=2.5):
+    """The plain per-call GPU form of the serving requests: for each request
+    of a prefix of the same schedule, back to back on one stream, one device
+    call that generates its coefficients (the same counter-based stream the
+    server uses), one zero fill of the padding and one coset-forward
+    transform of 2^(k+b) points.  Stops after the chunk that passes
+    `budget_s` seconds."""
+    import torch
+    from zntt_amd import workloads as W
+    arrivals = W.serve_arrivals(rate=args.rate, duration=args.duration, seed=4)
+    buf = torch.empty((1 << 25) * 4, dtype=torch.int64, device="cuda")  # 2^25 Fr values
+    stream = torch.cuda.current_stream()
+    sp = stream.cuda_stream
+    roots = {}
+
+    def call(i):
+        _, k, b = arrivals[i]
+        L = k + b
+        if L not in roots:
+            roots[L] = Z.primitive_root(field, L)
+        Z.fill_counter_device(ctx, field, 4, i, buf.data_ptr(), 1 << k, stream=sp)
+        buf[(1 << k) * 4:(1 << L) * 4].zero_()
+        ctx.execute_device(field, Z.COSET_FORWARD, L, buf.data_ptr(), roots[L], shift=shift, stream=sp)
+        return 1 << L
+
+    for i in range(min(8, len(arrivals))):  # warm-up: kernels and tables
+        call(i)
+    torch.cuda.synchronize()
+    done, elems, t0 = 0, 0, time.perf_counter()
+    while done < len(arrivals) and time.perf_counter() - t0 < budget_s:
+        for i in range(done, min(done + 50, len(arrivals))):
+            elems += call(i)
+        done = min(done + 50, len(arrivals))
+        torch.cuda.synchronize()
+    secs = time.perf_counter() - t0
+    del buf
+    return {"melem_s": elems / secs / 1e6, "requests": done, "seconds": secs,
+            "sample": f"the first {done} requests of the schedule, one after another on one stream: coefficient "
+                      f"fill, zero padding and one coset-forward device call each"}
+
+
 def run_serve(args, Z, ctx):
     from zntt_amd import serve
     from zntt_amd import workloads as W
@@ -683,6 +724,7 @@ def run_serve(args, Z, ctx):
     main_run = one_run(args.rate, args.duration)
     sub_rate = args.rate / 4
     sub = one_run(sub_rate, min(args.duration, 4.0))
+    per_call = serve_per_call(args, Z, ctx, field, shift)
     achieved = main_run["muls"] / (main_run["busy_ms"] * 1e-3)
     roofline = {"bound": "valu", "achieved": achieved / 1e12, "peak": VALU_MUL_PEAK / 1e12, "unit": "Tmul32/s",
                 "frac": achieved / VALU_MUL_PEAK, "traffic": None}
@@ -737,7 +779,8 @@ def run_serve(args, Z, ctx):
     line["warmup"] = 3
     line.update({"roofline": roofline, "cpu_baseline": cpu,
                  "extras": {"p50_ms": m["p50_ms"], "p99_ms": m["p99_ms"], "saturated": summary(m, args.rate),
-                            "sub_saturation": summary(sub, sub_rate), "cpu_all_cores": cpu_all}})
+                            "sub_saturation": summary(sub, sub_rate), "cpu_all_cores": cpu_all,
+                            "per_call": per_call, "speedup_vs_per_call": m["melem_s"] / per_call["melem_s"]}})
     return line
 
 
