@@ -160,3 +160,26 @@ def test_server_matches_per_call_lde(Z, ctx):
     coeffs = O.fill_counter(F, 4, 77, 1 << 14)
     assert np.array_equal(got, ctx.lde(F, coeffs, 14, 3, sh))
     srv.close()
+
+
+def test_large_requests_run_alone_and_match_oracle(Z, ctx):
+    """BN254 Fr requests of 2^20 points and more run each pass as a launch of
+    the single-transform pass kernel, whose first pass zero-pads the 2^k
+    coefficients; small requests share the same steps' level launches.
+    Every result equals the oracle's LDE, in the mix and alone."""
+    F = O.FIELD_FR
+    shifts = [O.from_int(F, 5), O.from_int(F, 7)]
+    reqs = [(18, 2, 0, 501), (17, 3, 1, 502), (10, 2, 0, 503), (20, 1, 1, 504), (12, 3, 1, 505), (19, 2, 0, 506)]
+    srv = Z.Server(ctx, F, hbm_budget=1 << 31, max_log_n=21)
+    mix = [srv.submit(k, b, shifts[si], seed=4, stream_id=sid, keep=True) for k, b, si, sid in reqs]
+    got = []
+    for t in mix:
+        srv.wait(t)
+        got.append(srv.read(t))
+    for (k, b, si, sid), g in zip(reqs, got):
+        assert np.array_equal(g, expected(F, 4, sid, k, b, shifts[si])), (k, b, si, sid)
+    k, b, si, sid = reqs[0]
+    t = srv.submit(k, b, shifts[si], seed=4, stream_id=sid, keep=True)
+    srv.wait(t)
+    assert np.array_equal(srv.read(t), got[0])
+    srv.close()

@@ -233,8 +233,10 @@ struct StageMap {
 // 2^lg / 2^LB columns of every lane, so consecutive threads touch consecutive
 // words.  Positions and twiddle indices use the column without the lane bits
 // (c.c0, c.lo0 count such columns); LDS slots use the full column.
+// PAD: the first pass reads zero at and past a.src_len, from the source and
+// from the coset table (a first pass that zero-pads its input).
 template <class F, int R, int l0, bool first_round, bool last_round, bool ODD, bool TWL = false, int N = 8,
-          bool WHOLE = false, int LB = 0>
+          bool WHOLE = false, int LB = 0, bool PAD = false>
 __device__ __forceinline__ void pass_round(typename F::E (&x)[N], const PassArgs &a, uint32_t *lds,
                                            const TileCtx &c) {
   using E = typename F::E;
@@ -295,7 +297,10 @@ __device__ __forceinline__ void pass_round(typename F::E (&x)[N], const PassArgs
           x[i * SET + k] = F::lds_get(lds, c.tile, slot_of(i, k));
         } else if (a.first) {
           const size_t src = ((size_t)brev_bits(mid, c.K) << c.hi_bits) + c.c0 + gr[i];
-          x[i * SET + k] = F::load_user(a.src, at(i, src));
+          if constexpr (PAD)
+            x[i * SET + k] = src < a.src_len ? F::load_user(a.src, at(i, src)) : F::zero();
+          else
+            x[i * SET + k] = F::load_user(a.src, at(i, src));
         } else {
           const size_t pos = (c.hi << (c.lo_bits + c.K)) + ((size_t)mid << c.lo_bits) + c.lo0 + gr[i];
           x[i * SET + k] = F::load_dev(a.src, at(i, pos));
@@ -311,7 +316,11 @@ __device__ __forceinline__ void pass_round(typename F::E (&x)[N], const PassArgs
       auto cos = [&](int e) {
         const int i = e / SET, k = e % SET;
         const int mid = base[i] | (k << l0);
-        return F::load_dev(a.coset, ((size_t)brev_bits(mid, c.K) << c.hi_bits) + c.c0 + gr[i]);
+        const size_t src = ((size_t)brev_bits(mid, c.K) << c.hi_bits) + c.c0 + gr[i];
+        if constexpr (PAD)
+          return src < a.src_len ? F::load_dev(a.coset, src) : F::zero();
+        else
+          return F::load_dev(a.coset, src);
       };
       sfor<N / 2>([&](auto h) {
         constexpr int e = 2 * decltype(h)::value;
@@ -389,7 +398,8 @@ __device__ __forceinline__ void pass_round(typename F::E (&x)[N], const PassArgs
 // Rounds of a K-stage pass over N = 2^LN elements a thread: the first takes
 // K - LN*(ceil(K/LN) - 1) stages, the rest LN each; unrolled at compile time
 // so `x` stays in registers.
-template <class F, int K, int RI, bool ODD, bool TWL = false, int N = 8, bool WHOLE = false, int LB = 0>
+template <class F, int K, int RI, bool ODD, bool TWL = false, int N = 8, bool WHOLE = false, int LB = 0,
+          bool PAD = false>
 __device__ __forceinline__ void pass_rounds(typename F::E (&x)[N], const PassArgs &a, uint32_t *lds,
                                             const TileCtx &c) {
   constexpr int LN = N == 8 ? 3 : 2;
@@ -397,16 +407,17 @@ __device__ __forceinline__ void pass_rounds(typename F::E (&x)[N], const PassArg
   constexpr int R0 = K - LN * (NR - 1);
   constexpr int R = RI == 0 ? R0 : LN;
   constexpr int L0 = RI == 0 ? 0 : R0 + LN * (RI - 1);
-  pass_round<F, R, L0, RI == 0, RI == NR - 1, ODD, TWL, N, WHOLE, LB>(x, a, lds, c);
+  pass_round<F, R, L0, RI == 0, RI == NR - 1, ODD, TWL, N, WHOLE, LB, PAD>(x, a, lds, c);
   if constexpr (RI + 1 < NR) {
     __syncthreads();
-    pass_rounds<F, K, RI + 1, ODD, TWL, N, WHOLE, LB>(x, a, lds, c);
+    pass_rounds<F, K, RI + 1, ODD, TWL, N, WHOLE, LB, PAD>(x, a, lds, c);
   }
 }
 
 // LB > 0: 2^LB interleaved transforms (see pass_round); a.log_g counts the
-// lane bits, a.L and a.s0 are those of one transform.
-template <class F, int K, bool ODD, int N = 8, int LB = 0>
+// lane bits, a.L and a.s0 are those of one transform.  PAD: a first pass
+// reads zero at and past a.src_len.
+template <class F, int K, bool ODD, int N = 8, int LB = 0, bool PAD = false>
 __global__ __launch_bounds__(512) void ntt_pass_r8_kernel(const PassArgs a) {
   using E = typename F::E;
   extern __shared__ uint32_t lds[];
@@ -428,7 +439,7 @@ __global__ __launch_bounds__(512) void ntt_pass_r8_kernel(const PassArgs a) {
     c.lo0 = (w & (((size_t)1 << lbb) - 1)) << lgr;
   }
   E x[N];
-  pass_rounds<F, K, 0, ODD, false, N, false, LB>(x, a, lds, c);
+  pass_rounds<F, K, 0, ODD, false, N, false, LB, PAD>(x, a, lds, c);
 }
 
 // Persistent launches claim items from ctr[0] and count finished workgroups

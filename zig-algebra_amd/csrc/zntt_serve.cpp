@@ -516,6 +516,43 @@ struct HipDevice : Device {
     r->out = nullptr;
   }
 
+  // BN254 Fr requests of at least 2^kSoloLog points run each pass as a launch
+  // of its own, with the single-transform pass kernel (zero-padding its
+  // first pass): one such transform fills the device, and that kernel is
+  // faster per butterfly than the level kernel's.  The others share the
+  // step's level launch.
+  static constexpr int kSoloLog = 20;
+  bool solo(const Request *r) const {
+    if (field != ZNTT_FIELD_BN254_FR || r->L < kSoloLog) return false;
+    for (int p = 0; p < r->npass; p++) {  // the single-transform kernel's shapes
+      const int K = r->plan.K[p], tl = K + r->plan.log_g[p];
+      if (K < 5 || K > 12 || tl < 9 || tl > 11) return false;
+    }
+    return true;
+  }
+
+  // The arguments of pass p of request r, as the level kernel derives them
+  // from the request's BatchXform.
+  static zntt::PassArgs pass_args(const Request *r, int p) {
+    zntt::PassArgs a = {};
+    int s0 = 1;
+    for (int q = 0; q < p; q++) s0 += r->plan.K[q];
+    a.first = p == 0;
+    a.last = p == r->npass - 1;
+    a.src = a.first ? (const uint32_t *)(r->in ? r->in : r->req.coeffs) : (const uint32_t *)r->scratch;
+    a.dst = a.last ? (uint32_t *)r->out : (uint32_t *)r->scratch;
+    a.tw = r->tw->table;
+    a.coset = a.first ? r->coset->table : nullptr;
+    a.has_scale = 0;
+    a.L = r->L;
+    a.s0 = s0;
+    a.K = r->plan.K[p];
+    a.log_g = r->plan.log_g[p];
+    a.src_len = (uint64_t)1 << r->req.log_n;
+    a.tw_one = 1;
+    return a;
+  }
+
   long submit_step(const std::vector<StepItem> &items) override {
     Slot *sl = nullptr;
     for (auto &x : slots)
@@ -524,14 +561,17 @@ struct HipDevice : Device {
         break;
       }
     if (!sl) return -ZNTT_DEVICE_ERROR;  // more steps in flight than slots: a scheduler bug
-    const size_t n = items.size();
+    size_t n = 0;  // items of the level launch
+    for (const auto &it : items) n += solo(it.r) ? 0 : 1;
     uint8_t *h = (uint8_t *)sl->host;
     memset(h, 0, 256);  // the step's counters (next item, finished workgroups) start at zero
     auto *hx = (zntt::BatchXform *)(h + 256);
     auto *he = (zntt::BatchEntry *)(h + 256 + align_up(sizeof(zntt::BatchXform) * n, 256));
     uint32_t first = 0, uniform = 0;
-    for (size_t i = 0; i < n; i++) {
-      const Request *r = items[i].r;
+    size_t i = 0;
+    for (const auto &item : items) {
+      const Request *r = item.r;
+      if (solo(r)) continue;
       zntt::BatchXform &x = hx[i];
       memset(&x, 0, sizeof x);
       x.src = (const uint32_t *)(r->in ? r->in : r->req.coeffs);
@@ -550,20 +590,25 @@ struct HipDevice : Device {
         x.tiles[p] = (uint32_t)(((size_t)1 << r->L) >> (r->plan.K[p] + r->plan.log_g[p]));
         s0 += r->plan.K[p];
       }
-      he[i] = zntt::BatchEntry{(uint32_t)i, (uint32_t)items[i].pass, first, 0};
-      const uint32_t tiles = x.tiles[items[i].pass];
+      he[i] = zntt::BatchEntry{(uint32_t)i, (uint32_t)item.pass, first, 0};
+      const uint32_t tiles = x.tiles[item.pass];
       if (i == 0) uniform = tiles;
       if (tiles != uniform) uniform = 0;
       first += tiles;
+      i++;
     }
     const size_t bytes = (uint8_t *)(he + n) - h;
     uint8_t *d = (uint8_t *)sl->dev;
     hipError_t e = hipEventRecord(sl->start, stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, stream);
-    if (e == hipSuccess)
-      e = zntt::launch_batch_level(field, (const zntt::BatchXform *)(d + 256),
-                                   (const zntt::BatchEntry *)(d + ((uint8_t *)he - h)), (uint32_t)n, first,
-                                   uniform, (uint32_t *)d, grid, tile_log, stream);
+    if (n > 0) {
+      if (e == hipSuccess) e = hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, stream);
+      if (e == hipSuccess)
+        e = zntt::launch_batch_level(field, (const zntt::BatchXform *)(d + 256),
+                                     (const zntt::BatchEntry *)(d + ((uint8_t *)he - h)), (uint32_t)n, first,
+                                     uniform, (uint32_t *)d, grid, tile_log, stream);
+    }
+    for (const auto &item : items)
+      if (e == hipSuccess && solo(item.r)) e = zntt::launch_pass_fr_padded(pass_args(item.r, item.pass), stream);
     if (e == hipSuccess) e = hipEventRecord(sl->end, stream);
     if (e != hipSuccess) return -hip_status(e);
     sl->id = next_id++;
