@@ -5,13 +5,19 @@ Default (the headline line): BN254 Fr forward NTT of 2^24 elements.  A step
 is one in-place forward NTT (libs/ntt/src/root.zig:65-90 semantics, root =
 primitive_root(24)) over 2^24 elements already resident in HBM, like the
 reference's own benchmark, which transforms the same buffer in place on every
-iteration (libs/pairing/src/bench.zig:16-24, 107-111).
+iteration (libs/pairing/src/bench.zig:16-24, 107-111).  The same line also
+carries the other two parts of the headline metric, measured after that
+timed region: the uneven batch's speedup over one plain device call per
+transform (extras.batched_vs_per_call_speedup) and the serving loop's p50 /
+p99 latency at Poisson 2000/s (extras.serve_p50_ms, extras.serve_p99_ms),
+each with its bit-exactness check (--headline-only skips them).
 
 Other configurations print their own JSON line when asked for:
   --workload gl-batch   4096 Goldilocks transforms of 2^12, one persistent launch
   --workload bb-batch / kb-batch  the same batch over BabyBear / KoalaBear
   --workload fr-uneven  512 BN254 Fr transforms, 2^10..2^20, forward / inverse /
-                        coset-forward, one persistent launch vs one call each
+                        coset-forward, one HIP graph replay (a persistent launch
+                        per pass level and per single-tile size) vs one call each
   --workload ref-2p20   the reference's own NTT benchmark: nttWithTwiddles over
                         2^20 BN254 Fr, input fromInt(i % 1000), in place
   --workload poly-mul   product of two 2^20-coefficient BN254 Fr polynomials by
@@ -329,10 +335,20 @@ def run_fr_uneven(args, Z, ctx):
     ms = timed(lambda: b.execute(stream=sp), args.steps, args.warmup, stream)
     b.status(stream=sp)
 
-    def per_call():
-        for (d, L, p, r, s) in items:
+    def per_call(its=items):
+        for (d, L, p, r, s) in its:
             ctx.execute_device(field, d, L, p, r, s, stream=sp)
     pc_ms = timed(per_call, max(2, args.steps // 5), 1, stream)
+    # Same inputs through both forms, once each: every result bit-equal.
+    xs = [torch.from_numpy(Z.fill_uniform(field, W.item_seed(3, i), 1 << L).view(np.int64)).cuda()
+          for i, (L, _) in enumerate(spec)]
+    for t, x in zip(bufs, xs):
+        t.copy_(x)
+    b.execute(stream=sp)
+    per_call([(d, L, x.data_ptr(), r, s2) for (d, L, _, r, s2), x in zip(items, xs)])
+    b.status(stream=sp)
+    bit_equal = all(torch.equal(t, x) for t, x in zip(bufs, xs))
+    del xs
     achieved = muls / (ms * 1e-3)
     roofline = {"bound": "valu", "achieved": achieved / 1e12, "peak": VALU_MUL_PEAK / 1e12, "unit": "Tmul32/s",
                 "frac": achieved / VALU_MUL_PEAK, "traffic": None}
@@ -370,13 +386,15 @@ def run_fr_uneven(args, Z, ctx):
     line = base_line("Melem/s BN254 Fr uneven batch (512 x 2^10..2^20, mixed directions)",
                      n_total / (ms * 1e-3) / 1e6, "Melem/s", args, ms,
                      {"workload": "512 BN254 Fr transforms, log2 size uniform 10..20, forward/inverse/"
-                                  "coset-forward, one persistent launch", "field": "bn254_fr",
+                                  "coset-forward, one HIP graph of persistent launches (one per pass level "
+                                  "and one per single-tile size)", "field": "bn254_fr",
                       "count": len(spec), "elements": n_total},
                      "synthetic: seed 3 metadata, per-transform seeded coefficients", "u32")
     line.update({"roofline": roofline, "cpu_baseline": cpu,
                  "extras": {"batch_shape": b.shape(), "autotune_ms": tuned_ms, "per_call_ms": pc_ms,
                             "per_call_melem_s": n_total / (pc_ms * 1e-3) / 1e6,
-                            "speedup_vs_per_call": pc_ms / ms, "cpu_all_cores": cpu_all}})
+                            "speedup_vs_per_call": pc_ms / ms, "bit_equal_to_per_call": bit_equal,
+                            "cpu_all_cores": cpu_all}})
     b.close()
     return line
 
@@ -699,14 +717,24 @@ def serve_per_call(args, Z, ctx, field, shift, budget_s=2.5):
                       f"fill, zero padding and one coset-forward device call each"}
 
 
-def run_serve(args, Z, ctx):
+def serve_samples(rate, duration):
+    """Oracle digests of sampled requests of the default schedule (Poisson
+    2000/s for 10 s, seed 4), by request index; none for other schedules."""
+    if rate != 2000.0 or duration != 10.0:
+        return {}
+    gold = json.load(open(os.path.join(ROOT, "tests", "golden", "oracle_digests.json"))).get("serve_lde", [])
+    return {c["index"]: c for c in gold}
+
+
+def run_serve(args, Z, ctx, sub_saturation=True, per_call_budget_s=20.0):
     from zntt_amd import serve
     from zntt_amd import workloads as W
     field = Z.FIELD_FR
     shift = Z.from_int(field, 5)
     budget = 64 << 30
+    samples = serve_samples(args.rate, args.duration)
 
-    def one_run(rate, duration):
+    def one_run(rate, duration, keep=()):
         arrivals = W.serve_arrivals(rate=rate, duration=duration, seed=4)
         srv = Z.Server(ctx, field, hbm_budget=budget, max_log_n=25)
         # Warm-up: one request of every size class (the kernels, the tables).
@@ -715,16 +743,28 @@ def run_serve(args, Z, ctx):
             srv.wait(t)
             srv.release(t)
         busy0 = srv.stats()["busy_ms"]
-        res = serve.replay(srv, arrivals, shift, seed=4)
+        res = serve.replay(srv, arrivals, shift, seed=4, keep=keep)
         res["busy_ms"] = res["stats"]["busy_ms"] - busy0
         res["muls"] = serve_muls(arrivals)
+        # Served results of the sampled requests against the oracle's digests.
+        checked = []
+        for i, tk in res["kept"].items():
+            got = srv.read(tk)
+            ok = hashlib.sha256(memoryview(got)).hexdigest() == samples[i]["evaluations"]
+            checked.append({"index": i, "log_n": samples[i]["log_n"], "log_blowup": samples[i]["log_blowup"],
+                            "bit_exact": ok})
+            del got
+            srv.release(tk)
+        res["checked"] = checked
         srv.close()
         return res
 
-    main_run = one_run(args.rate, args.duration)
+    main_run = one_run(args.rate, args.duration, keep=samples.keys())
     sub_rate = args.rate / 4
-    sub = one_run(sub_rate, min(args.duration, 4.0))
-    per_call = serve_per_call(args, Z, ctx, field, shift)
+    sub = one_run(sub_rate, min(args.duration, 4.0)) if sub_saturation else None
+    # The plain per-call form over the whole schedule when it fits the
+    # budget (then both legs ran the same requests), else over its prefix.
+    per_call = serve_per_call(args, Z, ctx, field, shift, budget_s=per_call_budget_s)
     achieved = main_run["muls"] / (main_run["busy_ms"] * 1e-3)
     roofline = {"bound": "valu", "achieved": achieved / 1e12, "peak": VALU_MUL_PEAK / 1e12, "unit": "Tmul32/s",
                 "frac": achieved / VALU_MUL_PEAK, "traffic": None}
@@ -779,9 +819,42 @@ def run_serve(args, Z, ctx):
     line["warmup"] = 3
     line.update({"roofline": roofline, "cpu_baseline": cpu,
                  "extras": {"p50_ms": m["p50_ms"], "p99_ms": m["p99_ms"], "saturated": summary(m, args.rate),
-                            "sub_saturation": summary(sub, sub_rate), "cpu_all_cores": cpu_all,
-                            "per_call": per_call, "speedup_vs_per_call": m["melem_s"] / per_call["melem_s"]}})
+                            "sub_saturation": summary(sub, sub_rate) if sub else None, "cpu_all_cores": cpu_all,
+                            "per_call": per_call, "speedup_vs_per_call": m["melem_s"] / per_call["melem_s"],
+                            "same_requests_as_per_call": per_call["requests"] == m["requests"],
+                            "sampled_results_vs_oracle": m["checked"],
+                            "bit_exact_sampled": all(c["bit_exact"] for c in m["checked"]) if m["checked"]
+                            else None}})
     return line
+
+
+def add_secondary_metrics(line, args, Z, ctx):
+    """The other two parts of the headline metric, measured in the same run
+    after the 2^24 transform's timed region: the uneven batch (512 BN254 Fr
+    transforms 2^10..2^20, mixed directions) against one plain device call per
+    transform, and the serving loop's p50/p99 latency at Poisson 2000/s with
+    its throughput against the plain per-call form of the same requests."""
+    sub = argparse.Namespace(**vars(args))
+    sub.skip_cpu = True
+    sub.steps, sub.warmup = 10, 2
+    print("bench: uneven batch leg", file=sys.stderr, flush=True)
+    u = run_fr_uneven(sub, Z, ctx)
+    print("bench: serving leg", file=sys.stderr, flush=True)
+    v = run_serve(sub, Z, ctx, sub_saturation=False)
+    ue, ve = u["extras"], v["extras"]
+    line["extras"]["batched_vs_per_call_speedup"] = ue["speedup_vs_per_call"]
+    line["extras"]["uneven_batch"] = {"melem_s": u["value"], "ms": u["ms_per_step"], "per_call_ms": ue["per_call_ms"],
+                                      "roofline_frac": u["roofline"]["frac"], "bit_equal_to_per_call":
+                                      ue.get("bit_equal_to_per_call"), "batch_shape": ue["batch_shape"]}
+    line["extras"]["serve_p50_ms"] = ve["p50_ms"]
+    line["extras"]["serve_p99_ms"] = ve["p99_ms"]
+    line["extras"]["serve"] = {"melem_s": v["value"], "requests": v["config"]["requests"],
+                               "rate_per_s": args.rate, "speedup_vs_per_call": ve["speedup_vs_per_call"],
+                               "same_requests_as_per_call": ve["same_requests_as_per_call"],
+                               "per_call_melem_s": ve["per_call"]["melem_s"],
+                               "roofline_frac": v["roofline"]["frac"],
+                               "bit_exact_sampled": ve["bit_exact_sampled"],
+                               "sampled_results_vs_oracle": ve["sampled_results_vs_oracle"]}
 
 
 def main() -> int:
@@ -799,6 +872,8 @@ def main() -> int:
     ap.add_argument("--no-tune", action="store_true", help="batch workloads: keep the default launch shape")
     ap.add_argument("--rate", type=float, default=2000.0, help="serve: request arrival rate (1/s)")
     ap.add_argument("--duration", type=float, default=10.0, help="serve: arrival window (s)")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="fr-ntt: skip the uneven-batch and serving legs of the headline metric")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -810,6 +885,8 @@ def main() -> int:
     ctx = Z.Context(0)
     if args.workload == "fr-ntt":
         line = run_fr_ntt(args, Z, ctx)
+        if not args.headline_only:
+            add_secondary_metrics(line, args, Z, ctx)
     elif args.workload == "gl-batch":
         line = run_gl_batch(args, Z, ctx)
     elif args.workload == "fr-uneven":

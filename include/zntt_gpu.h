@@ -98,12 +98,17 @@ typedef struct zntt_ctx zntt_ctx;
 
 /* hbm_budget_bytes: cap on device memory the context may hold (0 = 64 GiB). */
 int zntt_ctx_create(zntt_ctx **out, int device, uint64_t hbm_budget_bytes);
-/* Every batch and server made on `ctx` must be destroyed first: they use the
- * context (its lock, stream and twiddle cache) until they are. */
+/* Releases the caller's handle.  Batches and servers made on `ctx` hold a
+ * reference to it: while any is alive, the context's stream, lock and the
+ * twiddle tables they use stay valid (unused cache entries and buffers are
+ * freed at once), and the last zntt_batch_destroy / zntt_server_destroy
+ * frees the rest.  `ctx` itself must not be passed to any call afterwards. */
 void zntt_ctx_destroy(zntt_ctx *ctx);
 /* Device bytes currently held (scratch + twiddle cache + staging). */
 uint64_t zntt_ctx_bytes_in_use(const zntt_ctx *ctx);
-/* Drops every cached twiddle table and scratch buffer. */
+/* Frees every cached twiddle / coset table that no live batch or server
+ * uses (those stay, pinned), and the context's scratch and staging buffers.
+ * Waits for the context's stream and the device first. */
 int zntt_ctx_trim(zntt_ctx *ctx);
 const char *zntt_status_string(int status);
 /* Bytes per element of a field, 0 for an unknown id. */

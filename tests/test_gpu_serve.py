@@ -66,15 +66,14 @@ def mixed_requests(field, count, seed):
     return out, shifts
 
 
-# The Goldilocks case of the mixed-request test made a GPU memory fault when it
-# ran after the batch tests in one process, with a register-round level kernel
-# for batch and serving launches (since taken out).  Its cause was not found on
-# the CPU: an emulation of those kernels' launches under AddressSanitizer is
-# clean.  The case stays skipped until the cause is shown.
-MIXED = [O.FIELD_FR,
-         pytest.param(O.FIELD_GL, marks=pytest.mark.skip(
-             reason="GPU memory fault after the batch tests with a level kernel since removed; cause not yet shown")),
-         O.FIELD_BB, O.FIELD_KB]
+# The Goldilocks case once made a GPU memory fault when it ran after the batch
+# tests in one process.  The mistake found since: zntt_ctx_trim and
+# zntt_ctx_destroy freed twiddle tables that live batches and servers had
+# pinned, and a batch or server destroyed after its context wrote into the
+# freed context and cache entries (tests/test_host_lifetime.py shows each
+# order under AddressSanitizer).  Children now hold a reference on their
+# context and trims skip pinned entries.
+MIXED = ALL
 
 
 @pytest.mark.parametrize("field", MIXED, ids=IDS)
@@ -182,4 +181,37 @@ def test_large_requests_run_alone_and_match_oracle(Z, ctx):
     t = srv.submit(k, b, shifts[si], seed=4, stream_id=sid, keep=True)
     srv.wait(t)
     assert np.array_equal(srv.read(t), got[0])
+    srv.close()
+
+
+def test_config5_size_requests_match_oracle_digests(Z, ctx):
+    """Requests of the serving benchmark's own schedule at its largest sizes
+    (k 21 and 22, blowup 4 and 8: 2^23 .. 2^25 evaluations, the zero-padding
+    first pass with K = 8 and 9) and two of its first requests, all submitted
+    at once beside small generated requests: each result equals the oracle's
+    LDE digest (tests/golden/oracle_digests.json, "serve_lde")."""
+    import hashlib
+    import json
+    import os
+    F = O.FIELD_FR
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "oracle_digests.json")
+    with open(path) as f:
+        cases = json.load(f)["serve_lde"]
+    assert {(c["log_n"], c["log_blowup"]) for c in cases} >= {(21, 2), (21, 3), (22, 2), (22, 3)}
+    sh = O.from_int(F, 5)
+    srv = Z.Server(ctx, F, hbm_budget=16 << 30, max_log_n=25)
+    tickets = []
+    for j, c in enumerate(cases):
+        tickets.append(srv.submit(c["log_n"], c["log_blowup"], sh, seed=c["seed"], stream_id=c["stream_id"],
+                                  keep=True))
+        srv.submit(9 + j % 4, 2 + j % 2, sh, seed=5, stream_id=j)  # small neighbours, results not kept
+    for c, t in zip(cases, tickets):
+        srv.wait(t)
+        got = srv.read(t)
+        assert hex(int(got[0][0])) == c["head"][0], c["index"]
+        assert hashlib.sha256(memoryview(got)).hexdigest() == c["evaluations"], (c["index"], c["log_n"])
+        srv.release(t)
+        del got
+    srv.drain()
+    assert srv.stats()["failed"] == 0
     srv.close()

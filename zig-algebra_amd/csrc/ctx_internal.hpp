@@ -6,6 +6,7 @@
 #include <string.h>
 
 #include <array>
+#include <atomic>
 #include <list>
 #include <mutex>
 #include <vector>
@@ -46,6 +47,11 @@ struct CosetEntry {
 };
 
 struct zntt_ctx {
+  // References: one for the caller (dropped by zntt_ctx_destroy) and one per
+  // live batch or server made on the context.  The context, its stream and
+  // its cache are freed when the last reference goes, so children may
+  // outlive the caller's handle.
+  std::atomic<int> refs{1};
   int device = 0;
   hipStream_t stream = nullptr;
   uint64_t budget = 0;
@@ -79,6 +85,10 @@ struct CallGuard {
   std::lock_guard<std::mutex> lock;
   explicit CallGuard(zntt_ctx *c) : lock(c->mu) { c->protect_from = c->tick + 1; }
 };
+
+// Reference counting of the context (see zntt_ctx::refs).
+void ctx_retain(zntt_ctx *c);
+void ctx_release(zntt_ctx *c);
 
 bool field_ok(int field);
 size_t user_bytes(int field);

@@ -33,36 +33,6 @@
 
 namespace zntt {
 
-static int tile_log_for(int field) { return field == kFr ? 11 : 12; }  // LDS: 72 KiB Fr, 32 / 16 KiB others
-
-PassPlan plan_passes(int field, int L) {
-  const int tl = tile_log_for(field);
-  return plan_passes_shape(L, tl, tl - 2);  // keep >= 4 groups per tile for wide global accesses
-}
-
-PassPlan plan_passes_shape(int L, int tl, int kmax) {
-  PassPlan p = {};
-  p.tile_log = tl;
-  if (L <= tl) {
-    p.passes = 1;
-    p.K[0] = L;
-    p.log_g[0] = 0;
-    return p;
-  }
-  const int P = (L + kmax - 1) / kmax;
-  p.passes = P;
-  int s0 = 1;
-  for (int i = 0; i < P; i++) {
-    const int K = L / P + (i < L % P ? 1 : 0);
-    p.K[i] = K;
-    const int room = tl - K;
-    const int avail = (i == 0) ? (L - K) : (s0 - 1);
-    p.log_g[i] = room < avail ? room : avail;
-    s0 += K;
-  }
-  return p;
-}
-
 hipError_t launch_transform(int field, int L, uint32_t *data, uint32_t *scratch, const uint32_t *tw,
                             const uint32_t *coset, const uint32_t *scale_dev, bool tw_one, hipStream_t stream) {
   if (L == 0) return hipSuccess;

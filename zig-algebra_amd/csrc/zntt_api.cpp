@@ -191,6 +191,55 @@ int ensure_pinned(zntt_ctx *c, size_t bytes) {
   return ZNTT_OK;
 }
 
+// Frees every cache entry no batch or server has pinned (every entry with
+// `all`), and the context's reusable buffers.  Pinned entries stay: live
+// children read them.  Caller holds the context lock.
+void release_cache(zntt_ctx *c, bool all = false) {
+  hipStreamSynchronize(c->stream);
+  hipDeviceSynchronize();
+  for (auto it = c->tables.begin(); it != c->tables.end();) {
+    if (it->pins > 0 && !all) {
+      ++it;
+      continue;
+    }
+    dev_free(c, it->table, it->bytes);
+    dev_free(c, it->inv_table, it->inv_bytes);
+    it = c->tables.erase(it);
+  }
+  for (auto it = c->cosets.begin(); it != c->cosets.end();) {
+    if (it->pins > 0 && !all) {
+      ++it;
+      continue;
+    }
+    dev_free(c, it->table, it->bytes);
+    it = c->cosets.erase(it);
+  }
+  dev_free(c, c->scratch, c->scratch_bytes);
+  dev_free(c, c->staging, c->staging_bytes);
+  dev_free(c, c->aux, c->aux_bytes);
+  dev_free(c, c->m31_table, c->m31_table_bytes);
+  c->scratch = c->staging = c->aux = c->m31_table = nullptr;
+  c->scratch_bytes = c->staging_bytes = c->aux_bytes = c->m31_table_bytes = 0;
+  c->m31_log_n = -1;
+  if (c->pinned) hipHostFree(c->pinned);
+  c->pinned = nullptr;
+  c->pinned_bytes = 0;
+}
+
+void ctx_retain(zntt_ctx *c) { c->refs.fetch_add(1, std::memory_order_relaxed); }
+
+// Drops one reference; the last one frees everything (no child is left, so
+// nothing is pinned any more) and the context itself.
+void ctx_release(zntt_ctx *c) {
+  if (c->refs.fetch_sub(1, std::memory_order_acq_rel) != 1) return;
+  {
+    CallGuard g(c);
+    release_cache(c, true);
+  }
+  hipStreamDestroy(c->stream);
+  delete c;
+}
+
 std::vector<Elem> root_chain(int field, const Elem &root, int L) {
   std::vector<Elem> ch(L + 1);
   ch[0] = root;
@@ -470,33 +519,17 @@ int zntt_ctx_create(zntt_ctx **out, int device, uint64_t hbm_budget_bytes) {
 int zntt_ctx_trim(zntt_ctx *c) {
   if (!c) return ZNTT_INVALID_ARGUMENT;
   CallGuard g(c);
-  hipStreamSynchronize(c->stream);
-  hipDeviceSynchronize();
-  for (auto &e : c->tables) {
-    dev_free(c, e.table, e.bytes);
-    dev_free(c, e.inv_table, e.inv_bytes);
-  }
-  c->tables.clear();
-  for (auto &e : c->cosets) dev_free(c, e.table, e.bytes);
-  c->cosets.clear();
-  dev_free(c, c->scratch, c->scratch_bytes);
-  dev_free(c, c->staging, c->staging_bytes);
-  dev_free(c, c->aux, c->aux_bytes);
-  dev_free(c, c->m31_table, c->m31_table_bytes);
-  c->scratch = c->staging = c->aux = c->m31_table = nullptr;
-  c->scratch_bytes = c->staging_bytes = c->aux_bytes = c->m31_table_bytes = 0;
-  c->m31_log_n = -1;
-  if (c->pinned) hipHostFree(c->pinned);
-  c->pinned = nullptr;
-  c->pinned_bytes = 0;
+  release_cache(c);
   return ZNTT_OK;
 }
 
 void zntt_ctx_destroy(zntt_ctx *c) {
   if (!c) return;
-  zntt_ctx_trim(c);
-  hipStreamDestroy(c->stream);
-  delete c;
+  {
+    CallGuard g(c);
+    release_cache(c);
+  }
+  ctx_release(c);
 }
 
 uint64_t zntt_ctx_bytes_in_use(const zntt_ctx *c) { return c ? c->in_use : 0; }

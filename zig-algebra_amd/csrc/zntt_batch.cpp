@@ -305,6 +305,7 @@ int zntt_batch_create(zntt_ctx *c, const zntt_transform *ts, size_t count, zntt_
     delete b;
     return st;
   }
+  ctx_retain(c);  // the batch reads the context's stream and pinned tables until destroyed
   *out = b;
   return ZNTT_OK;
 }
@@ -428,11 +429,13 @@ int zntt_batch_shape(const zntt_batch *b, int *tile_log, int *kmax, int *wgs_per
 
 void zntt_batch_destroy(zntt_batch *b) {
   if (!b) return;
+  zntt_ctx *c = b->ctx;
   {
-    CallGuard g(b->ctx);
+    CallGuard g(c);
     release(b);
   }
   delete b;
+  ctx_release(c);
 }
 
 }  // extern "C"

@@ -701,6 +701,7 @@ int zntt_server_create(zntt_ctx *ctx, const zntt_server_config *cfg, zntt_server
     s->dev = std::move(d);
   }
   zntt_server *raw = s.release();
+  if (ctx) ctx_retain(ctx);  // released by zntt_server_destroy
   if (!cfg->simulate) raw->thread = std::thread(scheduler_main, raw);
   *out = raw;
   return ZNTT_OK;
@@ -882,7 +883,9 @@ void zntt_server_destroy(zntt_server *s) {
     s->all.clear();
   }
   s->dev.reset();
+  zntt_ctx *ctx = s->ctx;
   delete s;
+  if (ctx) ctx_release(ctx);
 }
 
 int zntt_fill_counter(zntt_ctx *c, int field, uint64_t seed, uint64_t stream_id, void *data, size_t n,

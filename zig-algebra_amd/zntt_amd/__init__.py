@@ -373,9 +373,10 @@ def fri_proof_sizes(field: int, log_domain: int, log_initial_degree: int, log_fi
 class Context:
     """One GPU: a HIP stream, scratch memory and the device twiddle cache.
 
-    Batches and servers made on a context hold it (zntt_ctx_destroy requires
-    them destroyed first), so close() closes those still open before it
-    releases the context."""
+    Batches and servers made on a context hold a reference to it in the
+    library, so they stay valid whatever order handles are released in;
+    close() also closes those still open, so that their device memory is
+    returned at a known point rather than whenever they are collected."""
 
     def __init__(self, device: int = 0, hbm_budget_bytes: int = 0):
         h = ctypes.c_void_p()

@@ -14,9 +14,13 @@ import time
 import numpy as np
 
 
-def replay(srv, arrivals, shift, seed: int = 4, spin_s: float = 2e-4) -> dict:
+def replay(srv, arrivals, shift, seed: int = 4, spin_s: float = 2e-4, keep=()) -> dict:
     """Submits every request of `arrivals` on time, waits for all of them and
-    returns the latency distribution, the makespan and the server's stats."""
+    returns the latency distribution, the makespan and the server's stats.
+    Requests whose index is in `keep` keep their results on the device; their
+    tickets are returned in ``kept`` (index -> ticket) for the caller to read
+    and release."""
+    keep = set(keep)
     tickets = []
     lag = []
     t0 = time.perf_counter()
@@ -29,18 +33,22 @@ def replay(srv, arrivals, shift, seed: int = 4, spin_s: float = 2e-4) -> dict:
             if due - now > spin_s:
                 time.sleep(due - now - spin_s)
         lag.append(time.perf_counter() - due)
-        tickets.append(srv.submit(k, b, shift, seed=seed, stream_id=i))
+        tickets.append(srv.submit(k, b, shift, seed=seed, stream_id=i, keep=i in keep))
     srv.drain()
     t_end = time.perf_counter()
     lat, queued, elems = [], [], 0
-    for tk, (_, k, b) in zip(tickets, arrivals):
+    kept = {}
+    for i, (tk, (_, k, b)) in enumerate(zip(tickets, arrivals)):
         q = srv.query(tk)
         if q["state"] != 2:
             raise RuntimeError(f"request {tk} ended in state {q['state']} (status {q['status']})")
         lat.append(q["latency_ms"])
         queued.append(q["queued_ms"])
         elems += 1 << (k + b)
-        srv.release(tk)
+        if i in keep:
+            kept[i] = tk
+        else:
+            srv.release(tk)
     lat = np.array(lat)
     queued = np.array(queued)
     makespan = t_end - t0
@@ -58,4 +66,5 @@ def replay(srv, arrivals, shift, seed: int = 4, spin_s: float = 2e-4) -> dict:
         "queued_p99_ms": float(np.percentile(queued, 99)) if len(queued) else None,
         "submit_lag_p99_ms": float(np.percentile(np.array(lag) * 1e3, 99)) if lag else None,
         "stats": st,
+        "kept": kept,
     }
