@@ -145,7 +145,12 @@ int zntt_free_twiddles(zntt_ctx *ctx, const void *const *stages, size_t num_stag
 /* nttWithTwiddles(F, data, log_n, twiddles) -- root.zig:148-175.
  * `stages`/`stage_lens`/`num_stages` are the Zig `[]const []const F`.  A table
  * registered by zntt_precompute_twiddles is read from the device cache; any
- * other (foreign) table is uploaded and used exactly as given. */
+ * other (foreign) table is uploaded and used exactly as given.  A registered
+ * table is found by its stage pointers and then checked against a
+ * fingerprint of sampled entries taken at registration: buffers freed
+ * without zntt_free_twiddles and reused for other content are detected, the
+ * stale registration dropped and the table used as a foreign one.  (A
+ * registered table edited in place must be registered again.) */
 int zntt_ntt_with_twiddles(zntt_ctx *ctx, int field, void *data, size_t len, size_t log_n,
                            const void *const *stages, const size_t *stage_lens, size_t num_stages);
 
@@ -180,7 +185,12 @@ int zntt_primitive_root(int field, size_t log_size, void *out);
 /* One transform on device-resident data (HBM), enqueued on `stream`, a
  * hipStream_t with HIP's meaning (NULL = the null stream; zntt_ctx_stream()
  * gives the context's own stream).  Asynchronous: returns
- * once the work is enqueued.  `root`/`shift` are host pointers to one
+ * once the work is enqueued.  Transforms above one tile (BN254 Fr 2^11, the
+ * other fields 2^12) use the context's scratch buffer between passes; a
+ * call on a different stream from the context's previous asynchronous call
+ * makes its stream wait for that call's work first (an event), so calls on
+ * several streams of one context are correct and run one after another on
+ * the device.  Use one context per stream for transforms that overlap.  `root`/`shift` are host pointers to one
  * element (shift only for ZNTT_COSET_FORWARD).  The twiddle table comes from
  * the device-resident cache (built on first use, on `stream`). */
 typedef struct {
@@ -193,6 +203,19 @@ typedef struct {
 } zntt_transform;
 
 int zntt_execute_device(zntt_ctx *ctx, const zntt_transform *t, void *stream);
+
+/* nttWithTwiddles / inttWithTwiddles (root.zig:148-175, :180-212) on
+ * device-resident data: `data` is a device pointer of `len` = 2^log_n
+ * elements, the table is the caller's host `[]const []const F` as for
+ * zntt_ntt_with_twiddles (registered: read from the device cache; foreign:
+ * uploaded and used as given), and the transform is enqueued on `stream`.
+ * Same checks in the same order as the host-buffer calls. */
+int zntt_ntt_with_twiddles_device(zntt_ctx *ctx, int field, void *data, size_t len, size_t log_n,
+                                  const void *const *stages, const size_t *stage_lens, size_t num_stages,
+                                  void *stream);
+int zntt_intt_with_twiddles_device(zntt_ctx *ctx, int field, void *data, size_t len, size_t log_n,
+                                   const void *const *stages, const size_t *stage_lens, size_t num_stages,
+                                   void *stream);
 
 /* ------------------------------------------------------------------------ */
 /* Batched transforms.  No reference counterpart.  All transforms of a batch */

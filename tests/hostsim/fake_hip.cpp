@@ -30,6 +30,8 @@ std::mutex g_mu;
 // While set, events report "not ready": work a test has enqueued stays in
 // flight (its launches have touched their buffers; completion is withheld).
 std::atomic<int> g_hold{0};
+// Calls to the runtime's allocators and frees, and stream waits on events.
+std::atomic<long> g_allocs{0}, g_waits{0};
 hipStream_t g_capturing = nullptr;
 std::vector<std::function<void()>> *g_recording = nullptr;
 
@@ -107,14 +109,17 @@ hipError_t hipGetLastError(void) { return hipSuccess; }
 hipError_t hipDeviceSynchronize(void) { return hipSuccess; }
 
 hipError_t hipMalloc(void **p, size_t bytes) {
+  g_allocs++;
   *p = aligned_alloc(256, (bytes + 255) & ~(size_t)255);
   return *p ? hipSuccess : hipErrorOutOfMemory;
 }
 hipError_t hipFree(void *p) {
+  g_allocs++;
   free(p);
   return hipSuccess;
 }
 hipError_t hipHostMalloc(void **p, size_t bytes, unsigned int) { return hipMalloc(p, bytes); }
+hipError_t hipMallocAsync(void **p, size_t bytes, hipStream_t) { return hipMalloc(p, bytes); }
 hipError_t hipHostFree(void *p) { return hipFree(p); }
 
 hipError_t hipMemcpy(void *dst, const void *src, size_t bytes, hipMemcpyKind) {
@@ -142,6 +147,7 @@ hipError_t hipEventCreate(hipEvent_t *e) {
   *e = (hipEvent_t)malloc(16);
   return hipSuccess;
 }
+hipError_t hipEventCreateWithFlags(hipEvent_t *e, unsigned int) { return hipEventCreate(e); }
 hipError_t hipEventDestroy(hipEvent_t e) {
   free((void *)e);
   return hipSuccess;
@@ -156,7 +162,10 @@ hipError_t hipEventElapsedTime(float *ms, hipEvent_t, hipEvent_t) {
   *ms = 0.01f;
   return hipSuccess;
 }
-hipError_t hipStreamWaitEvent(hipStream_t, hipEvent_t, unsigned int) { return hipSuccess; }
+hipError_t hipStreamWaitEvent(hipStream_t, hipEvent_t, unsigned int) {
+  g_waits++;
+  return hipSuccess;
+}
 
 hipError_t hipStreamBeginCapture(hipStream_t s, hipStreamCaptureMode) {
   std::lock_guard<std::mutex> lk(g_mu);
@@ -192,6 +201,8 @@ hipError_t hipGraphExecDestroy(hipGraphExec_t e) {
   return hipSuccess;
 }
 void fake_hip_hold(int on) { g_hold = on; }
+long fake_hip_alloc_calls() { return g_allocs; }
+long fake_hip_stream_waits() { return g_waits; }
 
 hipError_t hipGetSymbolAddress(void **p, const void *sym) {
   *p = (void *)sym;
@@ -288,6 +299,7 @@ hipError_t probe_peaks(double *mad, double *copy, hipStream_t) {
   return hipSuccess;
 }
 hipError_t configure_batch_lds() { return hipSuccess; }
+
 
 hipError_t launch_batch_level(int field, const BatchXform *xf, const BatchEntry *ent, uint32_t nent,
                               uint32_t nitems, uint32_t, uint32_t *counter, int, int, hipStream_t s) {

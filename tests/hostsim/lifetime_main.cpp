@@ -16,6 +16,8 @@
 #include "zntt_gpu.h"
 
 extern "C" void fake_hip_hold(int on);
+extern "C" long fake_hip_alloc_calls();
+extern "C" long fake_hip_stream_waits();
 
 #define CHECK(x)                                                              \
   do {                                                                        \
@@ -222,6 +224,33 @@ void late_destroy_after_new_ctx() {
   zntt_ctx_destroy(b);
 }
 
+// Multi-pass device transforms on two streams of one context: the second
+// stream waits once for the first one's work on the shared scratch; calls on
+// the same stream add no wait.
+void streams_share_scratch_in_order() {
+  zntt_ctx *c = nullptr;
+  CHECK(zntt_ctx_create(&c, 0, 1ull << 30));
+  hipStream_t a = nullptr, bs = nullptr;
+  (void)hipStreamCreateWithFlags(&a, 0);
+  (void)hipStreamCreateWithFlags(&bs, 0);
+  Bufs b({16, 16});
+  CHECK(zntt_execute_device(c, &b.ts[0], a));
+  const long w0 = fake_hip_stream_waits();
+  CHECK(zntt_execute_device(c, &b.ts[0], a));
+  EXPECT(fake_hip_stream_waits() == w0);
+  CHECK(zntt_execute_device(c, &b.ts[1], bs));
+  EXPECT(fake_hip_stream_waits() == w0 + 1);
+  CHECK(zntt_execute_device(c, &b.ts[1], bs));
+  EXPECT(fake_hip_stream_waits() == w0 + 1);
+  std::vector<uint8_t> host(((size_t)1 << 14) * 32, 0);
+  Root r = root_of(14);
+  CHECK(zntt_ntt(c, F, host.data(), (size_t)1 << 14, 14, r.b));  // the context's own stream waits too
+  EXPECT(fake_hip_stream_waits() == w0 + 2);
+  (void)hipStreamDestroy(a);
+  (void)hipStreamDestroy(bs);
+  zntt_ctx_destroy(c);
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
@@ -232,6 +261,7 @@ int main(int argc, char **argv) {
   else if (s == "server_outlives_ctx") server_outlives_ctx();
   else if (s == "trim_with_live_server") trim_with_live_server();
   else if (s == "late_destroy_after_new_ctx") late_destroy_after_new_ctx();
+  else if (s == "streams_share_scratch_in_order") streams_share_scratch_in_order();
   else return 1;
   printf("ok %s\n", s.c_str());
   return 0;

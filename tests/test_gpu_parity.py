@@ -374,3 +374,78 @@ def test_context_budget_exhaustion(Z):
         assert small.bytes_in_use() <= 1 << 20
     finally:
         small.close()
+
+
+# ---------------------------------------------------------------------------
+# Boundary: device-resident twiddled calls, stale registrations, streams
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("field", FIELDS, ids=IDS)
+def test_twiddled_device_matches_oracle(Z, ctx, field):
+    """nttWithTwiddles / inttWithTwiddles on device data, over a registered
+    table and over a foreign (non-power) one."""
+    import torch
+    sp = torch.cuda.current_stream().cuda_stream
+    for log_n in (9, 13, 17):
+        say("twiddled device", FNAME[field], log_n)
+        root = O.primitive_root(field, log_n)[1]
+        reg = ctx.precompute_twiddles(field, log_n, root)
+        st, otw = O.precompute_twiddles(field, log_n, root)
+        foreign = [O.gen_uniform(field, 300 + s, 1 << s) for s in range(log_n)]
+        x = O.gen_uniform(field, 400 + log_n, 1 << log_n)
+        for table, otable in ((reg, otw), (foreign, foreign)):
+            for inverse in (False, True):
+                exp = x.copy()
+                O.ntt_with_twiddles(field, exp, log_n, otable, inverse=inverse)
+                t = torch.from_numpy(x.view(np.int64).copy()).cuda()
+                ctx.with_twiddles_device(field, t.data_ptr(), log_n, table, inverse=inverse, stream=sp)
+                torch.cuda.synchronize()
+                got = t.cpu().numpy().view(np.uint64).reshape(exp.shape)
+                assert np.array_equal(got, exp), (log_n, inverse, table is reg)
+        reg.free()
+
+
+@pytest.mark.parametrize("field", [O.FIELD_FR, O.FIELD_GL], ids=["fr", "gl"])
+def test_stale_registration_is_not_used(Z, ctx, field):
+    """A registered table's host buffers reused for another table without
+    zntt_free_twiddles (same addresses, other content): the call uses the
+    buffers' current content, as the reference would."""
+    log_n = 12
+    root = O.primitive_root(field, log_n)[1]
+    tw = ctx.precompute_twiddles(field, log_n, root)
+    other = O.primitive_root(field, log_n + 1)[1]  # a root of another order: another table
+    st, otw = O.precompute_twiddles(field, log_n, other)
+    for s in range(log_n):
+        tw.stages[s][...] = otw[s]  # in place: the registered pointers now hold another table
+    x = O.gen_uniform(field, 77, 1 << log_n)
+    for inverse in (False, True):
+        exp = x.copy()
+        O.ntt_with_twiddles(field, exp, log_n, otw, inverse=inverse)
+        got = x.copy()
+        (ctx.intt_with_twiddles if inverse else ctx.ntt_with_twiddles)(field, got, log_n, tw)
+        assert np.array_equal(got, exp), inverse
+    tw.free()
+
+
+def test_multi_pass_transforms_on_two_streams(Z, ctx):
+    """Multi-pass device transforms of one context enqueued on two streams
+    without a host synchronisation between them: both results exact (the
+    shared scratch is ordered across the streams)."""
+    import torch
+    F = O.FIELD_FR
+    log_n = 17
+    root = O.primitive_root(F, log_n)[1]
+    xs = [O.gen_uniform(F, 1200 + i, 1 << log_n) for i in range(4)]
+    exps = []
+    for x in xs:
+        e = x.copy()
+        O.ntt(F, e, log_n, root)
+        exps.append(e)
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    ts = [torch.from_numpy(x.view(np.int64).copy()).cuda() for x in xs]
+    torch.cuda.synchronize()
+    for i, t in enumerate(ts):
+        ctx.execute_device(F, Z.FORWARD, log_n, t.data_ptr(), root, stream=streams[i % 2].cuda_stream)
+    torch.cuda.synchronize()
+    for t, e in zip(ts, exps):
+        assert np.array_equal(t.cpu().numpy().view(np.uint64).reshape(e.shape), e)

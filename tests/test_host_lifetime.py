@@ -16,6 +16,10 @@ or server made on it is alive and then keeps using that child:
 Before children held a reference on their context, every one of these was a
 heap-use-after-free (reads of freed twiddle tables by replayed launches,
 writes into freed cache entries and into the freed context).
+
+One more scenario uses the stand-in runtime's call counters: device
+transforms on two streams of one context wait for each other's use of the
+shared scratch exactly when the stream changes.
 """
 import os
 import shutil
@@ -27,7 +31,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 SIM = os.path.join(HERE, "hostsim")
 OUT = os.path.join(SIM, "_build")
 SCENARIOS = ["batch_outlives_ctx", "trim_with_live_batch", "server_outlives_ctx", "trim_with_live_server",
-             "late_destroy_after_new_ctx"]
+             "late_destroy_after_new_ctx", "streams_share_scratch_in_order"]
 
 
 @pytest.fixture(scope="module")

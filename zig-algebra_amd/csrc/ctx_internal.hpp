@@ -32,6 +32,7 @@ struct TwiddleEntry {
   uint32_t *inv_table = nullptr;     // inverse-with-twiddles table, built on demand
   size_t inv_bytes = 0;
   std::vector<const void *> key;     // host stage pointers of a registered table
+  uint64_t fingerprint = 0;          // of sampled entries of a registered table's host stages
   uint64_t last_use = 0;
   int pins = 0;                      // live batch plans reading this table
 };
@@ -75,15 +76,40 @@ struct zntt_ctx {
   size_t m31_table_bytes = 0;
   uint64_t m31_root = 0;
   int m31_log_n = -1;
+  // Ordering of the shared buffers' users across streams (CallGuard).
+  hipEvent_t shared_done = nullptr;
+  hipStream_t shared_stream = nullptr;
+  bool shared_used = false;
 };
 
 namespace zntt_detail {
 
 // Holds the context lock for one API call and shields the cache entries that
 // call touches from eviction by its own later allocations.
+//
+// Given the stream the call enqueues its work on, it also orders the call's
+// use of the context's shared device buffers (scratch, staging, aux, the
+// eight-lane M31 table): when the previous such call enqueued on another
+// stream, this stream first waits for the event that call recorded, and
+// this call records the event again when it ends.  So asynchronous calls on
+// different streams never overlap in those buffers, and a call on the same
+// stream as the last one adds only the record.
 struct CallGuard {
   std::lock_guard<std::mutex> lock;
-  explicit CallGuard(zntt_ctx *c) : lock(c->mu) { c->protect_from = c->tick + 1; }
+  zntt_ctx *ctx;
+  hipStream_t stream = nullptr;
+  bool ordered = false;
+  explicit CallGuard(zntt_ctx *c) : lock(c->mu), ctx(c) { c->protect_from = c->tick + 1; }
+  CallGuard(zntt_ctx *c, hipStream_t s) : lock(c->mu), ctx(c), stream(s), ordered(true) {
+    c->protect_from = c->tick + 1;
+    if (c->shared_used && c->shared_stream != s) (void)hipStreamWaitEvent(s, c->shared_done, 0);
+  }
+  ~CallGuard() {
+    if (!ordered) return;
+    (void)hipEventRecord(ctx->shared_done, stream);
+    ctx->shared_stream = stream;
+    ctx->shared_used = true;
+  }
 };
 
 // Reference counting of the context (see zntt_ctx::refs).

@@ -236,6 +236,7 @@ void ctx_release(zntt_ctx *c) {
     CallGuard g(c);
     release_cache(c, true);
   }
+  hipEventDestroy(c->shared_done);
   hipStreamDestroy(c->stream);
   delete c;
 }
@@ -400,22 +401,70 @@ int with_staging(zntt_ctx *c, int field, void *data, size_t n, Fn &&fn) {
   return hip_status(e);
 }
 
+// Fingerprint of a host twiddle table (stage s holds 2^s elements): a hash
+// of sampled entries -- entries 0, 1, the middle and the last of every
+// stage, and 64 evenly spaced entries of each of the three largest stages.
+// A registered table's host buffers freed and reused for other content (a
+// table of another root, data) change it with overwhelming probability.
+uint64_t table_fingerprint(int field, size_t log_n, const void *const *stages) {
+  const size_t ub = user_bytes(field);
+  uint64_t h = 0x9E3779B97F4A7C15ull ^ log_n;
+  auto mix = [&](const void *p) {
+    const uint8_t *b = (const uint8_t *)p;
+    for (size_t i = 0; i < ub; i += 8) {
+      uint64_t w;
+      memcpy(&w, b + i, 8);
+      h = (h ^ w) * 0xBF58476D1CE4E5B9ull;
+      h ^= h >> 31;
+    }
+  };
+  for (size_t s = 0; s < log_n; s++) {
+    const size_t len = (size_t)1 << s;
+    const uint8_t *st = (const uint8_t *)stages[s];
+    mix(st);
+    mix(st + (len - 1) * ub);
+    if (len > 2) {
+      mix(st + ub);
+      mix(st + (len / 2) * ub);
+    }
+    if (s + 3 >= log_n && len > 64)
+      for (size_t k = 0; k < 64; k++) mix(st + (k * (len / 64) + k % (len / 64)) * ub);
+  }
+  return h;
+}
+
+// The registered table for these host stages, if their content still
+// matches what was registered.  A registration whose buffers now hold other
+// content is stale (freed without zntt_free_twiddles and reused): it is
+// dropped, and the table is then used as a foreign one.
 TwiddleEntry *find_registered(zntt_ctx *c, int field, size_t log_n, const void *const *stages) {
-  for (auto &e : c->tables) {
+  for (auto it = c->tables.begin(); it != c->tables.end(); ++it) {
+    TwiddleEntry &e = *it;
     if (e.field != field || e.key.empty() || (size_t)e.L != log_n) continue;
     bool same = true;
     for (size_t s = 0; s < log_n && same; s++) same = e.key[s] == stages[s];
-    if (same) {
-      e.last_use = ++c->tick;
-      return &e;
+    if (!same) continue;
+    if (e.fingerprint != table_fingerprint(field, log_n, stages)) {
+      if (e.pins == 0) {
+        hipDeviceSynchronize();  // in-flight work may still read it
+        dev_free(c, e.table, e.bytes);
+        dev_free(c, e.inv_table, e.inv_bytes);
+        c->tables.erase(it);
+      } else {
+        e.key.clear();  // no longer reachable by its host pointers
+      }
+      return nullptr;
     }
+    e.last_use = ++c->tick;
+    return &e;
   }
   return nullptr;
 }
 
 // Uploads a caller's table (stage s holds 2^s user-layout elements) into the
 // dev layout at c->aux, followed by the trailing one.
-int upload_foreign(zntt_ctx *c, int field, size_t log_n, const void *const *stages, uint32_t **out) {
+int upload_foreign(zntt_ctx *c, int field, size_t log_n, const void *const *stages, hipStream_t s,
+                   uint32_t **out) {
   const size_t n = (size_t)1 << log_n;
   const size_t ub = user_bytes(field), db = dev_bytes(field);
   // aux holds: dev table (n entries) + user staging (n entries) + inverse table (n entries)
@@ -424,18 +473,21 @@ int upload_foreign(zntt_ctx *c, int field, size_t log_n, const void *const *stag
   uint32_t *dev = (uint32_t *)c->aux;
   uint8_t *user = (uint8_t *)c->aux + n * db;
   std::vector<uint8_t> host(n * ub);
-  for (size_t s = 0; s < log_n; s++) memcpy(host.data() + (((size_t)1 << s) - 1) * ub, stages[s], ((size_t)1 << s) * ub);
+  for (size_t k = 0; k < log_n; k++) memcpy(host.data() + (((size_t)1 << k) - 1) * ub, stages[k], ((size_t)1 << k) * ub);
   const Elem one = f_one(field);
   memcpy(host.data() + (n - 1) * ub, one.w, ub);
-  hipError_t e = hipMemcpyAsync(user, host.data(), n * ub, hipMemcpyHostToDevice, c->stream);
-  if (e == hipSuccess) e = zntt::launch_table_from_user(field, n, dev, (const uint32_t *)user, c->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  hipError_t e = hipMemcpyAsync(user, host.data(), n * ub, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = zntt::launch_table_from_user(field, n, dev, (const uint32_t *)user, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);  // `host` goes out of scope
   *out = dev;
   return hip_status(e);
 }
 
-int twiddled(zntt_ctx *c, int field, void *data, size_t len, size_t log_n, const void *const *stages,
-             const size_t *stage_lens, size_t num_stages, bool inverse) {
+// Checks a *_with_twiddles call's arguments in the reference's order
+// (root.zig:148-161, 180-193): LogTooLarge, LengthMismatch, the stage
+// count, then every stage length -- all before any data is touched.
+int check_twiddled(zntt_ctx *c, int field, const void *data, size_t len, size_t log_n, const void *const *stages,
+                   const size_t *stage_lens, size_t num_stages) {
   int st = check_transform_args(c, field, data, len, log_n);
   if (st) return st;
   if (num_stages != log_n) return ZNTT_INVALID_TWIDDLES;  // root.zig:152
@@ -444,47 +496,66 @@ int twiddled(zntt_ctx *c, int field, void *data, size_t len, size_t log_n, const
     if (stage_lens[s] != ((size_t)1 << s)) return ZNTT_INVALID_TWIDDLES;
   for (size_t s = 0; s < log_n; s++)
     if (!stages[s]) return ZNTT_INVALID_ARGUMENT;
-  CallGuard g(c);
-  if (log_n == 0) return ZNTT_OK;
+  return ZNTT_OK;
+}
+
+// The device table a *_with_twiddles call runs over: the registered copy of
+// `stages` (its derived inverse table for `inverse`), or the caller's table
+// uploaded as given.  `tw_one`: every stage of the result starts with 1.
+// Caller holds the context lock; log_n >= 1.
+int resolve_twiddles(zntt_ctx *c, int field, size_t log_n, const void *const *stages, bool inverse, hipStream_t s,
+                     const uint32_t **tw, bool *tw_one) {
   const int L = (int)log_n;
-  const uint32_t *tw = nullptr;
   // Registered tables are powers of a root, and derived inverse tables start
   // every stage with 1; a foreign forward table is used as given, so stages
   // may skip the multiplication by entry 0 only if each one holds 1.
-  bool tw_one = true;
+  *tw_one = true;
   TwiddleEntry *reg = find_registered(c, field, log_n, stages);
   if (reg) {
-    tw = reg->table;
+    *tw = reg->table;
     if (inverse) {
       if (!reg->inv_table) {
         reg->inv_bytes = reg->bytes;
-        st = dev_alloc(c, reg->inv_bytes, (void **)&reg->inv_table);
+        int st = dev_alloc(c, reg->inv_bytes, (void **)&reg->inv_table);
         if (st) return st;
-        hipError_t e = zntt::launch_inverse_table(field, L, reg->inv_table, reg->table, c->stream);
+        hipError_t e = zntt::launch_inverse_table(field, L, reg->inv_table, reg->table, s);
         if (e != hipSuccess) return hip_status(e);
       }
-      tw = reg->inv_table;
+      *tw = reg->inv_table;
     }
-  } else {
-    uint32_t *dev = nullptr;
-    st = upload_foreign(c, field, log_n, stages, &dev);
-    if (st) return st;
-    tw = dev;
-    const Elem one = f_one(field);
-    for (size_t s = 0; s < log_n && !inverse; s++)
-      if (!(load_elem(field, stages[s]) == one)) tw_one = false;
-    if (inverse) {
-      const size_t n = (size_t)1 << log_n;
-      uint32_t *inv = (uint32_t *)((uint8_t *)c->aux + n * (dev_bytes(field) + user_bytes(field)));
-      hipError_t e = zntt::launch_inverse_table(field, L, inv, dev, c->stream);
-      if (e != hipSuccess) return hip_status(e);
-      tw = inv;
-    }
+    return ZNTT_OK;
   }
+  uint32_t *dev = nullptr;
+  int st = upload_foreign(c, field, log_n, stages, s, &dev);
+  if (st) return st;
+  *tw = dev;
+  const Elem one = f_one(field);
+  for (size_t k = 0; k < log_n && !inverse; k++)
+    if (!(load_elem(field, stages[k]) == one)) *tw_one = false;
+  if (inverse) {
+    const size_t n = (size_t)1 << log_n;
+    uint32_t *inv = (uint32_t *)((uint8_t *)c->aux + n * (dev_bytes(field) + user_bytes(field)));
+    hipError_t e = zntt::launch_inverse_table(field, L, inv, dev, s);
+    if (e != hipSuccess) return hip_status(e);
+    *tw = inv;
+  }
+  return ZNTT_OK;
+}
+
+int twiddled(zntt_ctx *c, int field, void *data, size_t len, size_t log_n, const void *const *stages,
+             const size_t *stage_lens, size_t num_stages, bool inverse) {
+  int st = check_twiddled(c, field, data, len, log_n, stages, stage_lens, num_stages);
+  if (st) return st;
+  CallGuard g(c, c->stream);
+  if (log_n == 0) return ZNTT_OK;
+  const uint32_t *tw = nullptr;
+  bool tw_one = true;
+  st = resolve_twiddles(c, field, log_n, stages, inverse, c->stream, &tw, &tw_one);
+  if (st) return st;
   std::array<uint32_t, 9> scale;
   if (inverse) scale = f_to_dev(field, f_inv(field, f_from_u64(field, len)));  // root.zig:208-211
   return with_staging(c, field, data, len, [&](void *d) {
-    return run_device(c, field, L, d, tw, nullptr, inverse ? &scale : nullptr, c->stream, tw_one);
+    return run_device(c, field, (int)log_n, d, tw, nullptr, inverse ? &scale : nullptr, c->stream, tw_one);
   });
 }
 
@@ -509,6 +580,11 @@ int zntt_ctx_create(zntt_ctx **out, int device, uint64_t hbm_budget_bytes) {
   c->device = device;
   c->budget = hbm_budget_bytes ? hbm_budget_bytes : (64ull << 30);
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return ZNTT_DEVICE_ERROR;
+  }
+  if (hipEventCreateWithFlags(&c->shared_done, hipEventDisableTiming) != hipSuccess) {
+    hipStreamDestroy(c->stream);
     delete c;
     return ZNTT_DEVICE_ERROR;
   }
@@ -576,7 +652,7 @@ int zntt_bit_reverse(zntt_ctx *c, int field, void *data, size_t len) {
   if (!data) return ZNTT_INVALID_ARGUMENT;
   const int L = __builtin_ctzll(len);
   if (L > ZNTT_MAX_LOG_N) return ZNTT_UNSUPPORTED;
-  CallGuard g(c);
+  CallGuard g(c, c->stream);
   return with_staging(c, field, data, len, [&](void *d) {
     return hip_status(zntt::launch_bit_reverse(field, L, (uint32_t *)d, c->stream));
   });
@@ -586,7 +662,7 @@ int zntt_ntt(zntt_ctx *c, int field, void *data, size_t len, size_t log_n, const
   int st = check_transform_args(c, field, data, len, log_n);
   if (st) return st;
   if (!root) return ZNTT_INVALID_ARGUMENT;
-  CallGuard g(c);
+  CallGuard g(c, c->stream);
   if (log_n == 0) return ZNTT_OK;
   TwiddleEntry *t = nullptr;
   st = power_table(c, field, load_elem(field, root), (int)log_n, c->stream, &t);
@@ -600,7 +676,7 @@ int zntt_intt(zntt_ctx *c, int field, void *data, size_t len, size_t log_n, cons
   int st = check_transform_args(c, field, data, len, log_n);
   if (st) return st;
   if (!root) return ZNTT_INVALID_ARGUMENT;
-  CallGuard g(c);
+  CallGuard g(c, c->stream);
   if (log_n == 0) return ZNTT_OK;  // n^-1 = 1
   const Elem root_inv = f_inv(field, load_elem(field, root));  // root.zig:100
   TwiddleEntry *t = nullptr;
@@ -620,7 +696,7 @@ int zntt_precompute_twiddles(zntt_ctx *c, int field, size_t log_n, const void *r
   for (size_t s = 0; s < log_n; s++)
     if (!stages[s]) return ZNTT_INVALID_ARGUMENT;
   if (log_n > ZNTT_MAX_LOG_N) return ZNTT_OUT_OF_MEMORY;
-  CallGuard g(c);
+  CallGuard g(c, c->stream);
   if (log_n == 0) return ZNTT_OK;
   const int L = (int)log_n;
   // Drop a stale registration of the same host buffers.
@@ -663,6 +739,7 @@ int zntt_precompute_twiddles(zntt_ctx *c, int field, size_t log_n, const void *r
   }
   for (size_t s = 0; s < log_n; s++) memcpy(stages[s], host.data() + (((size_t)1 << s) - 1) * ub, ((size_t)1 << s) * ub);
   e.key.assign(stages, stages + log_n);
+  e.fingerprint = table_fingerprint(field, log_n, (const void *const *)stages);
   e.last_use = ++c->tick;
   c->tables.push_back(std::move(e));
   return ZNTT_OK;
@@ -695,12 +772,42 @@ int zntt_intt_with_twiddles(zntt_ctx *c, int field, void *data, size_t len, size
   return twiddled(c, field, data, len, log_n, stages, stage_lens, num_stages, true);
 }
 
+// nttWithTwiddles / inttWithTwiddles on device-resident data, enqueued on
+// `stream`: the same checks, tables and DAG as the host-buffer calls.
+static int twiddled_device(zntt_ctx *c, int field, void *data, size_t len, size_t log_n, const void *const *stages,
+                           const size_t *stage_lens, size_t num_stages, bool inverse, void *stream) {
+  int st = check_twiddled(c, field, data, len, log_n, stages, stage_lens, num_stages);
+  if (st) return st;
+  const hipStream_t s = (hipStream_t)stream;
+  CallGuard g(c, s);
+  if (log_n == 0) return ZNTT_OK;
+  const uint32_t *tw = nullptr;
+  bool tw_one = true;
+  st = resolve_twiddles(c, field, log_n, stages, inverse, s, &tw, &tw_one);
+  if (st) return st;
+  std::array<uint32_t, 9> scale;
+  if (inverse) scale = f_to_dev(field, f_inv(field, f_from_u64(field, len)));  // root.zig:208-211
+  return run_device(c, field, (int)log_n, data, tw, nullptr, inverse ? &scale : nullptr, s, tw_one);
+}
+
+int zntt_ntt_with_twiddles_device(zntt_ctx *c, int field, void *data, size_t len, size_t log_n,
+                                  const void *const *stages, const size_t *stage_lens, size_t num_stages,
+                                  void *stream) {
+  return twiddled_device(c, field, data, len, log_n, stages, stage_lens, num_stages, false, stream);
+}
+
+int zntt_intt_with_twiddles_device(zntt_ctx *c, int field, void *data, size_t len, size_t log_n,
+                                   const void *const *stages, const size_t *stage_lens, size_t num_stages,
+                                   void *stream) {
+  return twiddled_device(c, field, data, len, log_n, stages, stage_lens, num_stages, true, stream);
+}
+
 int zntt_coset_ntt(zntt_ctx *c, int field, void *data, size_t len, size_t log_n, const void *root,
                    const void *shift) {
   int st = check_transform_args(c, field, data, len, log_n);
   if (st) return st;
   if (!root || !shift) return ZNTT_INVALID_ARGUMENT;
-  CallGuard g(c);
+  CallGuard g(c, c->stream);
   TwiddleEntry *t = nullptr;
   if (log_n > 0) {
     st = power_table(c, field, load_elem(field, root), (int)log_n, c->stream, &t);
@@ -794,7 +901,7 @@ int zntt_lde(zntt_ctx *c, int field, const void *coeffs, size_t log_k, size_t lo
   if (L > (size_t)two_adicity(field)) return ZNTT_ORDER_TOO_LARGE;
   if (L > ZNTT_MAX_LOG_N) return ZNTT_UNSUPPORTED;
   const size_t k = (size_t)1 << log_k, n = (size_t)1 << L, ub = user_bytes(field);
-  CallGuard g(c);
+  CallGuard g(c, c->stream);
   TwiddleEntry *t = nullptr;
   const Elem root = f_primitive_root(field, (int)L);
   int st = ZNTT_OK;
@@ -832,7 +939,7 @@ int zntt_poly_mul(zntt_ctx *c, int field, const void *a, size_t a_len, const voi
   if (m > out_cap) return ZNTT_DEGREE_TOO_LARGE;  // poly.zig:199-200
   if (!out) return ZNTT_INVALID_ARGUMENT;
   const size_t ub = user_bytes(field);
-  CallGuard g(c);
+  CallGuard g(c, c->stream);
   void *dev = nullptr;
   int st = ensure(c, &c->aux, &c->aux_bytes, (na + nb + m) * ub);
   if (st) return st;
@@ -853,7 +960,7 @@ int zntt_poly_mul(zntt_ctx *c, int field, const void *a, size_t a_len, const voi
 int zntt_poly_mul_device(zntt_ctx *c, int field, const void *a, size_t a_len, const void *b, size_t b_len,
                          void *out, void *stream) {
   if (!c || !field_ok(field) || !a || !b || !out || a_len == 0 || b_len == 0) return ZNTT_INVALID_ARGUMENT;
-  CallGuard g(c);
+  CallGuard g(c, (hipStream_t)stream);
   return poly_mul_device(c, field, a, a_len, b, b_len, out, (hipStream_t)stream);
 }
 
@@ -865,7 +972,7 @@ int zntt_execute_device(zntt_ctx *c, const zntt_transform *t, void *stream) {
   if (t->direction == ZNTT_COSET_FORWARD && !t->shift) return ZNTT_INVALID_ARGUMENT;
   if (t->direction < ZNTT_FORWARD || t->direction > ZNTT_COSET_FORWARD) return ZNTT_INVALID_ARGUMENT;
   hipStream_t s = (hipStream_t)stream;  // HIP's convention: NULL is the null stream
-  CallGuard g(c);
+  CallGuard g(c, s);
   const int field = t->field, L = (int)t->log_n;
   const size_t n = (size_t)1 << L;
   Elem root = load_elem(field, t->root);
@@ -893,7 +1000,7 @@ int zntt_execute_device(zntt_ctx *c, const zntt_transform *t, void *stream) {
 
 int zntt_debug_field_op(zntt_ctx *c, int field, int op, const void *a, const void *b, void *out, size_t n) {
   if (!c || !field_ok(field) || !a || !b || !out || op < 0 || op > 2) return ZNTT_INVALID_ARGUMENT;
-  CallGuard g(c);
+  CallGuard g(c, c->stream);
   const size_t bytes = n * user_bytes(field);
   void *buf = nullptr;
   int st = dev_alloc(c, 3 * bytes, &buf);
@@ -955,7 +1062,7 @@ int zntt_fill_uniform(int field, uint64_t seed, void *out, size_t n) {
 
 int zntt_probe_peaks(zntt_ctx *c, double *mad_per_s, double *copy_bytes_per_s) {
   if (!c || !mad_per_s || !copy_bytes_per_s) return ZNTT_INVALID_ARGUMENT;
-  CallGuard g(c);
+  CallGuard g(c, c->stream);
   return hip_status(zntt::probe_peaks(mad_per_s, copy_bytes_per_s, c->stream));
 }
 

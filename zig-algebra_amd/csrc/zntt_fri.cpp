@@ -264,7 +264,7 @@ int zntt_fri_prove(zntt_ctx *c, int field, const zntt_fri_config *cfg, const znt
     return ZNTT_INVALID_ARGUMENT;
   if (!proof->roots || !proof->residual || !proof->pair_index || !proof->values || !proof->paths)
     return ZNTT_INVALID_ARGUMENT;
-  CallGuard g(c);
+  CallGuard g(c, (hipStream_t)stream);
   return prove(c, field, cfg, ops, transcript, evals, evals_len, proof, (hipStream_t)stream);
 }
 

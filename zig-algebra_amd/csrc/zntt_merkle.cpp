@@ -41,7 +41,7 @@ int zntt_merkle_commit(zntt_ctx *c, int field, const void *evals, size_t log_n, 
   if (log_n == 0) return ZNTT_INVALID_LENGTH;  // a layer needs an antipodal pair
   if (log_n > ZNTT_MAX_LOG_N) return ZNTT_UNSUPPORTED;
   const int leaf_log = (int)log_n - 1;
-  CallGuard g(c);
+  CallGuard g(c, (hipStream_t)stream);
   return commit(c, leaf_log, nodes, root, (hipStream_t)stream, [&](uint32_t *heap, bool full) {
     return zntt::launch_merkle_layer(field, evals, leaf_log, heap, full, (hipStream_t)stream);
   });
@@ -54,7 +54,7 @@ int zntt_merkle_from_hashes(zntt_ctx *c, const void *hashes, size_t count, void 
   while (((size_t)1 << cap_log) < count) {
     if (++cap_log > ZNTT_MAX_LOG_N) return ZNTT_UNSUPPORTED;
   }
-  CallGuard g(c);
+  CallGuard g(c, (hipStream_t)stream);
   return commit(c, cap_log, nodes, root, (hipStream_t)stream, [&](uint32_t *heap, bool full) {
     return zntt::launch_merkle_from_hashes(hashes, count, cap_log, heap, full, (hipStream_t)stream);
   });

@@ -95,7 +95,7 @@ int vec8_host(zntt_ctx *c, bool inverse, void *data, size_t len, size_t log_n, c
   if (log_n > ZNTT_VEC8_MAX_LOG_N) return ZNTT_UNSUPPORTED;
   uint64_t rv;
   memcpy(&rv, root, 8);
-  CallGuard g(c);
+  CallGuard g(c, c->stream);
   if ((st = ensure(c, &c->staging, &c->staging_bytes, len * 8))) return st;
   hipError_t e = hipMemcpyAsync(c->staging, data, len * 8, hipMemcpyHostToDevice, c->stream);
   if (e != hipSuccess) return hip_status(e);
@@ -124,7 +124,7 @@ int zntt_vec8_m31_device(zntt_ctx *c, int inverse, void *data, size_t log_n, con
   if (log_n > ZNTT_VEC8_MAX_LOG_N) return ZNTT_UNSUPPORTED;
   uint64_t rv;
   memcpy(&rv, root, 8);
-  CallGuard g(c);
+  CallGuard g(c, (hipStream_t)stream);
   return vec8_device(c, inverse != 0, data, (int)log_n, rv, (hipStream_t)stream);
 }
 

@@ -85,7 +85,6 @@ class DegreeTooLarge(ZnttError):
 class InvalidParameters(ZnttError):
     status = 12
 
-
 _ERRORS = {c.status: c for c in (LogTooLarge, LengthMismatch, InvalidLength, InvalidTwiddles, OutOfMemory,
                                  DeviceError, NoDevice, InvalidArgument, Unsupported, OrderTooLarge,
                                  DegreeTooLarge, InvalidParameters)}
@@ -106,6 +105,7 @@ EXPORTED = [
     "zntt_fri_proof_sizes", "zntt_fri_prove", "zntt_transcript_create", "zntt_transcript_destroy",
     "zntt_transcript_absorb_bytes", "zntt_transcript_challenge_bytes", "zntt_blake3",
     "zntt_ntt_vec8_m31", "zntt_intt_vec8_m31", "zntt_vec8_m31_device",
+    "zntt_ntt_with_twiddles_device", "zntt_intt_with_twiddles_device",
 ]
 
 M31_MODULUS = 0x7FFFFFFF
@@ -198,6 +198,8 @@ def _load():
     L.zntt_ntt_with_twiddles.argtypes = [vp, i32, vp, sz, sz, pp, ctypes.POINTER(sz), sz]
     L.zntt_intt_with_twiddles.argtypes = [vp, i32, vp, sz, sz, pp, ctypes.POINTER(sz), sz]
     L.zntt_coset_ntt.argtypes = [vp, i32, vp, sz, sz, vp, vp]
+    L.zntt_ntt_with_twiddles_device.argtypes = [vp, i32, vp, sz, sz, pp, ctypes.POINTER(sz), sz, vp]
+    L.zntt_intt_with_twiddles_device.argtypes = [vp, i32, vp, sz, sz, pp, ctypes.POINTER(sz), sz, vp]
     L.zntt_lde.argtypes = [vp, i32, vp, sz, sz, vp, vp]
     L.zntt_primitive_root.argtypes = [i32, sz, vp]
     L.zntt_execute_device.argtypes = [vp, ctypes.POINTER(Transform), vp]
@@ -442,12 +444,27 @@ class Context:
         self._twiddled(lib.zntt_intt_with_twiddles, field, data, log_n, twiddles)
 
     def _twiddled(self, fn, field, data, log_n, twiddles):
+        ptrs, lens, k = self._table_args(twiddles)
+        check(fn(self._h, field, _ptr(data), data.shape[0], log_n, ptrs, lens, k))
+
+    @staticmethod
+    def _table_args(twiddles):
         stages = twiddles.stages if isinstance(twiddles, Twiddles) else [np.ascontiguousarray(s) for s in twiddles]
         k = len(stages)
         ptrs = (ctypes.c_void_p * max(k, 1))(*[s.ctypes.data for s in stages])
         lens = (ctypes.c_size_t * max(k, 1))(*[s.shape[0] for s in stages])
-        check(fn(self._h, field, _ptr(data), data.shape[0], log_n,
-                 ctypes.cast(ptrs, ctypes.POINTER(ctypes.c_void_p)), lens, k))
+        return ctypes.cast(ptrs, ctypes.POINTER(ctypes.c_void_p)), lens, k
+
+    def with_twiddles_device(self, field: int, data_ptr: int, log_n: int, twiddles, inverse: bool = False,
+                             stream: int | None = None, length: int | None = None) -> None:
+        """nttWithTwiddles / inttWithTwiddles on 2^log_n device elements at
+        data_ptr, enqueued on `stream` (None: the context's own stream)."""
+        if stream is None:
+            stream = lib.zntt_ctx_stream(self._h)
+        ptrs, lens, k = self._table_args(twiddles)
+        fn = lib.zntt_intt_with_twiddles_device if inverse else lib.zntt_ntt_with_twiddles_device
+        n = (1 << log_n) if length is None else length
+        check(fn(self._h, field, data_ptr, n, log_n, ptrs, lens, k, stream))
 
     # ---- beyond zig-ntt ----------------------------------------------------
     def coset_ntt(self, field: int, data: np.ndarray, log_n: int, root, shift) -> None:
