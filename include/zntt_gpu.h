@@ -68,7 +68,8 @@ enum zntt_status {
   ZNTT_UNSUPPORTED = 9,       /* a size this device path does not implement      */
   ZNTT_ORDER_TOO_LARGE = 10,  /* error.OrderTooLarge    roots.zig:150 (LDE/roots) */
   ZNTT_DEGREE_TOO_LARGE = 11, /* error.DegreeTooLarge   libs/poly/src/poly.zig:200 */
-  ZNTT_INVALID_PARAMETERS = 12 /* FriError.InvalidParameters libs/fri/src/root.zig:77 */
+  ZNTT_INVALID_PARAMETERS = 12, /* FriError.InvalidParameters libs/fri/src/root.zig:77 */
+  ZNTT_NOT_PREPARED = 13       /* zntt_execute_device_noalloc: a table it needs is not cached */
 };
 
 enum zntt_field {
@@ -203,6 +204,27 @@ typedef struct {
 } zntt_transform;
 
 int zntt_execute_device(zntt_ctx *ctx, const zntt_transform *t, void *stream);
+
+/* No counterpart in the reference.  For callers that cannot allocate
+ * device memory -- code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc: zntt_execute_device with no allocation of
+ * any kind (no hipMalloc, hipFree or allocating library call).
+ *   - The twiddle table of (field, root, 2^log_n) -- of root^-1 for
+ *     ZNTT_INVERSE -- and, for ZNTT_COSET_FORWARD, the shift's coset table
+ *     must already be in the context's cache, put there by an earlier call
+ *     such as zntt_execute_device on the same transform; otherwise
+ *     ZNTT_NOT_PREPARED, with nothing enqueued.
+ *   - Transforms above one tile keep their intermediate passes in a static
+ *     device buffer of ZNTT_NOALLOC_SCRATCH_BYTES per GPU (2^log_n * 36
+ *     bytes BN254 Fr, 8 Goldilocks, 4 BabyBear / KoalaBear: up to 2^20,
+ *     2^23 and 2^24 elements); larger ones fail with ZNTT_OUT_OF_MEMORY
+ *     before anything is enqueued.
+ *   - Every such call on one GPU shares that buffer: calls whose work
+ *     overlaps on the device (on different streams, or from different
+ *     contexts on the same GPU) race.  Calls on one stream are ordered.
+ * Results equal zntt_execute_device's bit for bit. */
+#define ZNTT_NOALLOC_SCRATCH_BYTES (64u << 20)
+int zntt_execute_device_noalloc(zntt_ctx *ctx, const zntt_transform *t, void *stream);
 
 /* nttWithTwiddles / inttWithTwiddles (root.zig:148-175, :180-212) on
  * device-resident data: `data` is a device pointer of `len` = 2^log_n

@@ -23,6 +23,7 @@
 
 #include "batch.hpp"
 #include "kernels.hpp"
+#include "zntt_gpu.h"
 
 namespace {
 
@@ -299,7 +300,11 @@ hipError_t probe_peaks(double *mad, double *copy, hipStream_t) {
   return hipSuccess;
 }
 hipError_t configure_batch_lds() { return hipSuccess; }
-
+hipError_t noalloc_scratch_init(int) { return hipSuccess; }
+void *noalloc_scratch_ptr(int) {
+  static std::vector<uint8_t> buf(ZNTT_NOALLOC_SCRATCH_BYTES);
+  return buf.data();
+}
 
 hipError_t launch_batch_level(int field, const BatchXform *xf, const BatchEntry *ent, uint32_t nent,
                               uint32_t nitems, uint32_t, uint32_t *counter, int, int, hipStream_t s) {

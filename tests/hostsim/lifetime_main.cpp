@@ -224,6 +224,50 @@ void late_destroy_after_new_ctx() {
   zntt_ctx_destroy(b);
 }
 
+// zntt_execute_device_noalloc makes no allocation or free of any kind once
+// the tables are cached; it refuses sizes past its static scratch and
+// transforms whose tables are not cached, before enqueuing anything.
+void noalloc_allocates_nothing() {
+  zntt_ctx *c = nullptr;
+  CHECK(zntt_ctx_create(&c, 0, 1ull << 30));
+  Bufs b({16, 13, 15, 20});  // forward 2^16, inverse 2^13, coset 2^15, forward 2^20
+  for (auto &t : b.ts) CHECK(zntt_execute_device(c, &t, nullptr));  // builds and caches the tables
+  const long before = fake_hip_alloc_calls();
+  const uint64_t bytes = zntt_ctx_bytes_in_use(c);
+  for (int rep = 0; rep < 2; rep++)
+    for (auto &t : b.ts) CHECK(zntt_execute_device_noalloc(c, &t, nullptr));
+  EXPECT(fake_hip_alloc_calls() == before);
+  EXPECT(zntt_ctx_bytes_in_use(c) == bytes);
+  // Fr 2^21 needs 72 MiB of scratch, more than the static 64 MiB.
+  void *big = nullptr;
+  if (hipMalloc(&big, ((size_t)1 << 21) * 32) != hipSuccess) exit(4);
+  Root r21 = root_of(21);
+  zntt_transform t = {};
+  t.field = F;
+  t.direction = ZNTT_FORWARD;
+  t.log_n = 21;
+  t.data = big;
+  t.root = r21.b;
+  EXPECT(zntt_execute_device_noalloc(c, &t, nullptr) == ZNTT_OUT_OF_MEMORY);
+  // A root whose table is not cached.
+  Root other = small(12345);
+  t.log_n = 12;
+  t.root = other.b;
+  const long before2 = fake_hip_alloc_calls();
+  EXPECT(zntt_execute_device_noalloc(c, &t, nullptr) == ZNTT_NOT_PREPARED);
+  t.root = b.roots[1].b;  // cached for the inverse only: the forward table of that root is another one
+  t.log_n = 13;
+  t.direction = ZNTT_COSET_FORWARD;
+  t.shift = nullptr;
+  EXPECT(zntt_execute_device_noalloc(c, &t, nullptr) == ZNTT_INVALID_ARGUMENT);
+  Root sh7 = small(7);
+  t.shift = sh7.b;
+  EXPECT(zntt_execute_device_noalloc(c, &t, nullptr) == ZNTT_NOT_PREPARED);
+  EXPECT(fake_hip_alloc_calls() == before2);
+  (void)hipFree(big);
+  zntt_ctx_destroy(c);
+}
+
 // Multi-pass device transforms on two streams of one context: the second
 // stream waits once for the first one's work on the shared scratch; calls on
 // the same stream add no wait.
@@ -261,6 +305,7 @@ int main(int argc, char **argv) {
   else if (s == "server_outlives_ctx") server_outlives_ctx();
   else if (s == "trim_with_live_server") trim_with_live_server();
   else if (s == "late_destroy_after_new_ctx") late_destroy_after_new_ctx();
+  else if (s == "noalloc_allocates_nothing") noalloc_allocates_nothing();
   else if (s == "streams_share_scratch_in_order") streams_share_scratch_in_order();
   else return 1;
   printf("ok %s\n", s.c_str());

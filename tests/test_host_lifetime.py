@@ -17,9 +17,11 @@ Before children held a reference on their context, every one of these was a
 heap-use-after-free (reads of freed twiddle tables by replayed launches,
 writes into freed cache entries and into the freed context).
 
-One more scenario uses the stand-in runtime's call counters: device
-transforms on two streams of one context wait for each other's use of the
-shared scratch exactly when the stream changes.
+Two more scenarios use the stand-in runtime's call counters:
+* zntt_execute_device_noalloc makes no allocation or free once its tables
+  are cached, and refuses oversized or unprepared transforms up front;
+* device transforms on two streams of one context wait for each other's use
+  of the shared scratch exactly when the stream changes.
 """
 import os
 import shutil
@@ -31,7 +33,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 SIM = os.path.join(HERE, "hostsim")
 OUT = os.path.join(SIM, "_build")
 SCENARIOS = ["batch_outlives_ctx", "trim_with_live_batch", "server_outlives_ctx", "trim_with_live_server",
-             "late_destroy_after_new_ctx", "streams_share_scratch_in_order"]
+             "late_destroy_after_new_ctx", "noalloc_allocates_nothing", "streams_share_scratch_in_order"]
 
 
 @pytest.fixture(scope="module")

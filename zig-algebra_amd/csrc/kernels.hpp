@@ -134,6 +134,11 @@ hipError_t launch_transform_m31_lanes(int L, uint32_t *data, uint32_t *scratch, 
 // Elementwise device field op (0 add, 1 sub, 2 mul) on user-layout arrays.
 hipError_t launch_field_op(int field, int op, size_t n, const uint32_t *a, const uint32_t *b, uint32_t *out,
                            hipStream_t stream);
+// The static device scratch of zntt_execute_device_noalloc
+// (ZNTT_NOALLOC_SCRATCH_BYTES a GPU, noalloc_scratch.hip): init looks up its
+// address on `device`, which must be current; ptr returns it (null before).
+hipError_t noalloc_scratch_init(int device);
+void *noalloc_scratch_ptr(int device);
 // Microbenchmarks: v_mad_u64_u32 rate (mul/s) and device copy bandwidth (B/s).
 hipError_t probe_peaks(double *mad_per_s, double *copy_bytes_per_s, hipStream_t stream);
 

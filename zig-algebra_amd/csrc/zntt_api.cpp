@@ -357,6 +357,25 @@ int coset_table(zntt_ctx *c, int field, const Elem &shift, size_t count, hipStre
   return ZNTT_OK;
 }
 
+// Lookups of the two caches that never build or allocate (the no-allocation
+// entry point): the entry power_table / coset_table would return, or null.
+TwiddleEntry *find_power_table(zntt_ctx *c, int field, const Elem &root, int L) {
+  for (auto &e : c->tables)
+    if (e.field == field && e.key.empty() && e.L >= L && e.chain[e.L - L] == root) {
+      e.last_use = ++c->tick;
+      return &e;
+    }
+  return nullptr;
+}
+const uint32_t *find_coset_table(zntt_ctx *c, int field, const Elem &shift, size_t count) {
+  for (auto &e : c->cosets)
+    if (e.field == field && e.count >= count && e.shift == shift) {
+      e.last_use = ++c->tick;
+      return e.table;
+    }
+  return nullptr;
+}
+
 int check_transform_args(zntt_ctx *c, int field, const void *data, size_t len, size_t log_n) {
   if (!c || !field_ok(field)) return ZNTT_INVALID_ARGUMENT;
   if (log_n >= 64) return ZNTT_LOG_TOO_LARGE;         // root.zig:33-36
@@ -588,6 +607,15 @@ int zntt_ctx_create(zntt_ctx **out, int device, uint64_t hbm_budget_bytes) {
     delete c;
     return ZNTT_DEVICE_ERROR;
   }
+  // The no-allocation entry point's scratch on this device (looked up once
+  // per device, with the device current).
+  if (zntt::noalloc_scratch_init(device) != hipSuccess) {
+    hipGetLastError();
+    hipEventDestroy(c->shared_done);
+    hipStreamDestroy(c->stream);
+    delete c;
+    return ZNTT_DEVICE_ERROR;
+  }
   *out = c;
   return ZNTT_OK;
 }
@@ -632,6 +660,7 @@ const char *zntt_status_string(int status) {
     case ZNTT_ORDER_TOO_LARGE: return "OrderTooLarge";
     case ZNTT_DEGREE_TOO_LARGE: return "DegreeTooLarge";
     case ZNTT_INVALID_PARAMETERS: return "InvalidParameters";
+    case ZNTT_NOT_PREPARED: return "NotPrepared";
     default: return "unknown";
   }
 }
@@ -996,6 +1025,39 @@ int zntt_execute_device(zntt_ctx *c, const zntt_transform *t, void *stream) {
   }
   if (L == 0) return ZNTT_OK;
   return run_device(c, field, L, t->data, tw->table, cos, scale_p, s);
+}
+
+int zntt_execute_device_noalloc(zntt_ctx *c, const zntt_transform *t, void *stream) {
+  if (!c || !t || !field_ok(t->field) || !t->root) return ZNTT_INVALID_ARGUMENT;
+  if (t->log_n >= 64) return ZNTT_LOG_TOO_LARGE;
+  if (t->log_n > ZNTT_MAX_LOG_N) return ZNTT_UNSUPPORTED;
+  if (!t->data) return ZNTT_INVALID_ARGUMENT;
+  if (t->direction == ZNTT_COSET_FORWARD && !t->shift) return ZNTT_INVALID_ARGUMENT;
+  if (t->direction < ZNTT_FORWARD || t->direction > ZNTT_COSET_FORWARD) return ZNTT_INVALID_ARGUMENT;
+  const int field = t->field, L = (int)t->log_n;
+  const size_t n = (size_t)1 << L;
+  // Scratch need of a multi-pass transform against the static buffer.
+  const bool multi = L > 0 && zntt::plan_passes(field, L).passes > 1;
+  if (multi && n * dev_bytes(field) > (size_t)ZNTT_NOALLOC_SCRATCH_BYTES) return ZNTT_OUT_OF_MEMORY;
+  uint32_t *scratch = (uint32_t *)zntt::noalloc_scratch_ptr(c->device);
+  if (!scratch) return ZNTT_DEVICE_ERROR;
+  const hipStream_t s = (hipStream_t)stream;
+  CallGuard g(c);  // the cache's lock; the static scratch is this call's alone only on one stream
+  Elem root = load_elem(field, t->root);
+  std::array<uint32_t, 9> scale;
+  const uint32_t *scale_p = nullptr;
+  if (t->direction == ZNTT_INVERSE) {
+    root = f_inv(field, root);
+    scale = f_to_dev(field, f_inv(field, f_from_u64(field, n)));
+    scale_p = scale.data();
+  }
+  const TwiddleEntry *tw = nullptr;
+  if (L > 0 && !(tw = find_power_table(c, field, root, L))) return ZNTT_NOT_PREPARED;
+  const uint32_t *cos = nullptr;
+  if (t->direction == ZNTT_COSET_FORWARD && !(cos = find_coset_table(c, field, load_elem(field, t->shift), n)))
+    return ZNTT_NOT_PREPARED;
+  if (L == 0) return ZNTT_OK;
+  return hip_status(zntt::launch_transform(field, L, (uint32_t *)t->data, scratch, tw->table, cos, scale_p, true, s));
 }
 
 int zntt_debug_field_op(zntt_ctx *c, int field, int op, const void *a, const void *b, void *out, size_t n) {

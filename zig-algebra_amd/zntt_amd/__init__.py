@@ -85,9 +85,14 @@ class DegreeTooLarge(ZnttError):
 class InvalidParameters(ZnttError):
     status = 12
 
+
+class NotPrepared(ZnttError):
+    status = 13
+
+
 _ERRORS = {c.status: c for c in (LogTooLarge, LengthMismatch, InvalidLength, InvalidTwiddles, OutOfMemory,
                                  DeviceError, NoDevice, InvalidArgument, Unsupported, OrderTooLarge,
-                                 DegreeTooLarge, InvalidParameters)}
+                                 DegreeTooLarge, InvalidParameters, NotPrepared)}
 
 # Every symbol include/zntt_gpu.h declares.
 EXPORTED = [
@@ -105,7 +110,7 @@ EXPORTED = [
     "zntt_fri_proof_sizes", "zntt_fri_prove", "zntt_transcript_create", "zntt_transcript_destroy",
     "zntt_transcript_absorb_bytes", "zntt_transcript_challenge_bytes", "zntt_blake3",
     "zntt_ntt_vec8_m31", "zntt_intt_vec8_m31", "zntt_vec8_m31_device",
-    "zntt_ntt_with_twiddles_device", "zntt_intt_with_twiddles_device",
+    "zntt_ntt_with_twiddles_device", "zntt_intt_with_twiddles_device", "zntt_execute_device_noalloc",
 ]
 
 M31_MODULUS = 0x7FFFFFFF
@@ -203,6 +208,7 @@ def _load():
     L.zntt_lde.argtypes = [vp, i32, vp, sz, sz, vp, vp]
     L.zntt_primitive_root.argtypes = [i32, sz, vp]
     L.zntt_execute_device.argtypes = [vp, ctypes.POINTER(Transform), vp]
+    L.zntt_execute_device_noalloc.argtypes = [vp, ctypes.POINTER(Transform), vp]
     L.zntt_ctx_synchronize.argtypes = [vp]
     L.zntt_ctx_stream.argtypes = [vp]
     L.zntt_ctx_stream.restype = vp
@@ -563,6 +569,18 @@ class Context:
         s = None if shift is None else _elem(field, shift)
         t = Transform(field, direction, log_n, data_ptr, r.ctypes.data, None if s is None else s.ctypes.data)
         check(lib.zntt_execute_device(self._h, ctypes.byref(t), stream))
+
+    def execute_device_noalloc(self, field: int, direction: int, log_n: int, data_ptr: int, root, shift=None,
+                               stream: int | None = None) -> None:
+        """execute_device without any device allocation: the tables must be
+        cached already (NotPrepared otherwise) and intermediate passes use the
+        library's static per-GPU scratch (OutOfMemory beyond it)."""
+        if stream is None:
+            stream = lib.zntt_ctx_stream(self._h)
+        r = _elem(field, root)
+        s = None if shift is None else _elem(field, shift)
+        t = Transform(field, direction, log_n, data_ptr, r.ctypes.data, None if s is None else s.ctypes.data)
+        check(lib.zntt_execute_device_noalloc(self._h, ctypes.byref(t), stream))
 
     def debug_field_op(self, field: int, op: int, a: np.ndarray, b: np.ndarray) -> np.ndarray:
         out = np.zeros_like(a)
