@@ -252,9 +252,12 @@ typedef struct zntt_batch zntt_batch;
  * zntt_batch_autotune for the same sizes and directions is reused.  Sizes
  * run from 2^3 to 2^ZNTT_MAX_LOG_N (ZNTT_UNSUPPORTED otherwise). */
 int zntt_batch_create(zntt_ctx *ctx, const zntt_transform *ts, size_t count, zntt_batch **out);
-/* Enqueues every transform of the batch on `stream` as one persistent kernel
- * launch per pass level (at most four), replayed from a HIP graph captured on
- * the first call.  Do not run two executions of one batch at once (they share
+/* Enqueues every transform of the batch on `stream`, replayed from a HIP
+ * graph captured on the first call: per pass level (at most four), one
+ * persistent launch per kernel family of the level (the register-round
+ * level kernel per pass length and parity, the generic level kernel, the
+ * whole-transform kernel per size), the launches of a level on parallel
+ * branches of the graph.  Do not run two executions of one batch at once (they share
  * the scratch and work counters). */
 int zntt_batch_execute(zntt_batch *b, void *stream);
 /* Waits for `stream` and reports a HIP error raised by the batch's launches. */
@@ -264,8 +267,11 @@ int zntt_batch_status(zntt_batch *b, void *stream);
  * and for later batches of the same signature.  `best_ms`: its time per
  * execution.  The caller's data is not modified. */
 int zntt_batch_autotune(zntt_batch *b, int reps, double *best_ms);
-/* The batch's current shape. */
-int zntt_batch_shape(const zntt_batch *b, int *tile_log, int *kmax, int *wgs_per_cu, int *order);
+/* The batch's current shape.  level_kernel: 1 when passes with a full tile
+ * run in the register-round level kernel (one launch per pass length and
+ * parity of the first stage), 0 when every pass runs in the generic one. */
+int zntt_batch_shape(const zntt_batch *b, int *tile_log, int *kmax, int *wgs_per_cu, int *order,
+                     int *level_kernel);
 void zntt_batch_destroy(zntt_batch *b);
 
 /* ------------------------------------------------------------------------ */
@@ -273,9 +279,12 @@ void zntt_batch_destroy(zntt_batch *b);
 /* (the reference's STARK example evaluates its LDE by Horner's rule,        */
 /* examples/stark_prover.zig:117-129).  Requests are queued as they are      */
 /* submitted, admitted while the HBM budget and the active-request limit     */
-/* allow, and run by continuous batching: every scheduler step is one        */
-/* persistent launch holding the next pass of every admitted request, so a   */
-/* request admitted while others are on the device joins the next step.      */
+/* allow, and run by continuous batching: every scheduler step holds the     */
+/* next pass of every admitted request, grouped into a few persistent        */
+/* launches (requests that fit one tile by size; passes of larger requests   */
+/* by pass length and parity, so large requests share launches; the rest     */
+/* together), run as parallel branches of the server's stream.  A request    */
+/* admitted while others are on the device joins the next step.              */
 /* A request's result equals zntt_lde on the same coefficients.              */
 /* ------------------------------------------------------------------------ */
 typedef struct zntt_server zntt_server;

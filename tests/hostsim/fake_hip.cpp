@@ -44,7 +44,8 @@ struct FakeGraph {
 hipError_t run_or_record(hipStream_t s, std::function<void()> fn) {
   {
     std::lock_guard<std::mutex> lk(g_mu);
-    if (g_capturing && s == g_capturing) {
+    (void)s;  // branch streams forked from the captured stream are captured too
+    if (g_capturing) {
       g_recording->push_back(std::move(fn));
       return hipSuccess;
     }
@@ -229,20 +230,6 @@ hipError_t launch_transform(int field, int L, uint32_t *data, uint32_t *scratch,
   });
 }
 
-hipError_t launch_pass_fr_padded(const PassArgs &a, hipStream_t s) {
-  const PassArgs c = a;
-  return run_or_record(s, [=] {
-    const size_t n = (size_t)1 << c.L;
-    if (c.first)
-      rd(c.src, (size_t)c.src_len * 32);
-    else
-      rd(c.src, n * 36);
-    wr(c.dst, n * (c.last ? 32 : 36));
-    rd(c.tw, (n - 1) * 36);
-    if (c.coset) rd(c.coset, (size_t)c.src_len * 36);
-  });
-}
-
 hipError_t launch_build_table(int field, int L, uint32_t *flat, const uint32_t *pows, const uint32_t *one_dev,
                               hipStream_t s) {
   return run_or_record(s, [=] {
@@ -318,6 +305,12 @@ hipError_t launch_batch_level(int field, const BatchXform *xf, const BatchEntry 
     if (items != nitems) abort();  // the work list must cover exactly the launch's items
     wr(counter, 2 * sizeof(uint32_t));
   });
+}
+
+hipError_t launch_batch_level_r8(int field, int, bool, int, const BatchXform *xf, const BatchEntry *ent,
+                                 uint32_t nent, uint32_t nitems, uint32_t uniform, uint32_t *counter, int grid,
+                                 hipStream_t s) {
+  return launch_batch_level(field, xf, ent, nent, nitems, uniform, counter, grid, 0, s);
 }
 
 static hipError_t whole(int field, const BatchXform *xf, const BatchEntry *ent, uint32_t n, uint32_t *counter,

@@ -146,29 +146,7 @@ __global__ __launch_bounds__(BLOCK, WAVES_PER_SIMD) void batch_kernel(const Batc
     // this item's work; it is published through s_item at the end.
     uint32_t claimed = 0;
     if (threadIdx.x == 0) claimed = atomicAdd(next, 1u);
-    // Find the entry holding item `it`.  Equal tile counts: a division.
-    // Otherwise each step has every thread test one candidate boundary,
-    // narrowing [lo, hi) by a factor of the workgroup size; two LDS slots
-    // alternate so that a write never races a read of the step before.
-    const uint32_t nthreads = blockDim.x;
-    uint32_t lo = 0, hi = nent;
-    int ping = 0;
-    if (uniform_tiles) {
-      lo = it / uniform_tiles;
-      hi = lo + 1;
-    }
-    while (hi - lo > 1) {
-      const uint32_t stride = (hi - lo + nthreads - 1) / nthreads;
-      const uint32_t t = lo + threadIdx.x * stride;
-      if (t < hi) {
-        const uint32_t nx = t + stride;
-        if (ent[t].first <= it && (nx >= hi || ent[nx].first > it)) s_pos[ping] = t;
-      }
-      __syncthreads();
-      lo = s_pos[ping];
-      hi = min(lo + stride, hi);
-      ping ^= 1;
-    }
+    const uint32_t lo = find_entry(ent, nent, it, uniform_tiles, s_pos);
     const BatchEntry e = ent[lo];
     const BatchXform &X = xf[e.xf];
     const int p = (int)e.pass;
@@ -256,6 +234,16 @@ hipError_t launch_batch_level(int field, const BatchXform *xf, const BatchEntry 
                          uniform_tiles, counter);
   }
   return hipGetLastError();
+}
+
+hipError_t launch_batch_level_r8(int field, int K, bool odd, int tile_log, const BatchXform *xf,
+                                 const BatchEntry *ent, uint32_t nent, uint32_t nitems, uint32_t uniform_tiles,
+                                 uint32_t *counter, int grid, hipStream_t s) {
+  if (!level_r8_ok(field, K, tile_log)) return hipErrorInvalidValue;
+  if (uniform_tiles && (uint64_t)uniform_tiles * nent != nitems) return hipErrorInvalidValue;
+  if (field == kFr) return launch_level_r8_fr(K, odd, tile_log, xf, ent, nent, nitems, uniform_tiles, counter, grid, s);
+  if (field == kGl) return launch_level_r8_gl(K, odd, tile_log, xf, ent, nent, nitems, uniform_tiles, counter, grid, s);
+  return launch_level_r8_f31(field, K, odd, tile_log, xf, ent, nent, nitems, uniform_tiles, counter, grid, s);
 }
 
 hipError_t configure_batch_lds() {

@@ -53,11 +53,6 @@ hipError_t launch_transform(int field, int L, uint32_t *data, uint32_t *scratch,
 // Per-field pass launchers (pass_fr.hip, pass_gl.hip).
 hipError_t launch_transform_fr(int L, uint32_t *data, uint32_t *scratch, const uint32_t *tw, const uint32_t *coset,
                                const uint32_t *scale_dev, bool tw_one, hipStream_t stream);
-// One pass of a BN254 Fr transform, with a first pass that reads zero at and
-// past a.src_len from its source and its coset table (the serving loop's
-// large requests keep only their 2^k coefficients).  The pass must have the
-// register-round shape: 5 <= K <= 12 and a tile of 512 to 2048 elements.
-hipError_t launch_pass_fr_padded(const PassArgs &a, hipStream_t s);
 hipError_t launch_transform_gl(int L, uint32_t *data, uint32_t *scratch, const uint32_t *tw, const uint32_t *coset,
                                const uint32_t *scale_dev, bool tw_one, hipStream_t stream);
 hipError_t launch_transform_f31(int field, int L, uint32_t *data, uint32_t *scratch, const uint32_t *tw,

@@ -454,6 +454,104 @@ __device__ __forceinline__ void retire_workgroup(uint32_t *ctr) {
   }
 }
 
+// The entry of a persistent launch's work list (entries ordered by `first`,
+// strictly increasing from 0) that holds item `it`; every thread of the
+// workgroup calls it and gets the same index.  Equal tile counts: a
+// division.  Otherwise each step has every thread test one candidate
+// boundary, narrowing [lo, hi) by a factor of the workgroup size; two LDS
+// slots alternate so that a write never races a read of the step before.
+__device__ __forceinline__ uint32_t find_entry(const BatchEntry *ent, uint32_t nent, uint32_t it,
+                                               uint32_t uniform_tiles, uint32_t *s_pos) {
+  const uint32_t nthreads = blockDim.x;
+  uint32_t lo = 0, hi = nent;
+  int ping = 0;
+  if (uniform_tiles) {
+    lo = it / uniform_tiles;
+    hi = lo + 1;
+  }
+  while (hi - lo > 1) {
+    const uint32_t stride = (hi - lo + nthreads - 1) / nthreads;
+    const uint32_t t = lo + threadIdx.x * stride;
+    if (t < hi) {
+      const uint32_t nx = t + stride;
+      if (ent[t].first <= it && (nx >= hi || ent[nx].first > it)) s_pos[ping] = t;
+    }
+    __syncthreads();
+    lo = s_pos[ping];
+    hi = min(lo + stride, hi);
+    ping ^= 1;
+  }
+  return lo;
+}
+
+// Persistent kernel over the tiles of one pass level of a batch whose passes
+// all have K stages, first stages of parity ODD and tiles of blockDim.x * N
+// elements (N = round_elems<F>()): the single-transform pass kernel's
+// register rounds, with compile-time round positions, run per claimed tile.
+// Items are claimed from next[0] as in the other persistent launches
+// (next[0..2) zero at launch and again when it ends).  A first pass reads
+// zero at and past its transform's src_len, from the source and from the
+// coset table.
+template <class F, int K, bool ODD>
+__global__ __launch_bounds__(512) void level_r8_kernel(const BatchXform *__restrict__ xf,
+                                                       const BatchEntry *__restrict__ ent, uint32_t nent,
+                                                       uint32_t nitems, uint32_t uniform_tiles, uint32_t *next) {
+  using E = typename F::E;
+  constexpr int N = round_elems<F>();
+  extern __shared__ uint32_t lds[];
+  __shared__ uint32_t s_item, s_pos[2];
+  if (threadIdx.x == 0) s_item = atomicAdd(next, 1u);
+  __syncthreads();
+  uint32_t it = __builtin_amdgcn_readfirstlane(s_item);
+  while (it < nitems) {
+    uint32_t claimed = 0;
+    if (threadIdx.x == 0) claimed = atomicAdd(next, 1u);
+    const uint32_t ei = __builtin_amdgcn_readfirstlane(find_entry(ent, nent, it, uniform_tiles, s_pos));
+    const BatchEntry e = ent[ei];
+    const BatchXform &X = xf[e.xf];
+    const int p = (int)e.pass;
+    const uint32_t tile_id = it - e.first;
+    PassArgs a;
+    a.first = p == 0;
+    a.last = p == X.npass - 1;
+    a.src = a.first ? X.src : X.scratch;
+    a.dst = a.last ? X.data : X.scratch;
+    a.src_len = X.src_len;
+    a.tw_one = 1;  // batches use tables of powers
+    a.tw = X.tw;
+    a.coset = a.first ? X.coset : nullptr;
+    a.has_scale = a.last ? X.has_scale : 0;
+#pragma unroll
+    for (int k = 0; k < 9; k++) a.scale[k] = X.scale[k];
+    a.L = X.L;
+    a.s0 = X.s0[p];
+    a.K = K;
+    a.log_g = X.lg[p];
+    TileCtx c;
+    c.K = K;
+    c.lg = a.log_g;
+    c.G = 1 << c.lg;
+    c.tile = c.G << K;
+    c.lo_bits = a.s0 - 1;
+    c.hi_bits = a.L - c.lo_bits - K;
+    c.hi = c.lo0 = c.c0 = 0;
+    if (a.first) {
+      c.c0 = (size_t)tile_id << c.lg;
+    } else {
+      const int lbb = c.lo_bits - c.lg;
+      c.hi = (size_t)tile_id >> lbb;
+      c.lo0 = ((size_t)tile_id & (((size_t)1 << lbb) - 1)) << c.lg;
+    }
+    E x[N];
+    pass_rounds<F, K, 0, ODD, false, N, false, 0, true>(x, a, lds, c);
+    __syncthreads();  // LDS and s_pos are reused by the next item
+    if (threadIdx.x == 0) s_item = claimed;
+    __syncthreads();
+    it = __builtin_amdgcn_readfirstlane(s_item);
+  }
+  if (threadIdx.x == 0) retire_workgroup(next);
+}
+
 // Waves a SIMD that the whole-transform kernel asks the register allocator
 // for: two 512-thread workgroups a CU (the LDS allows two for every field)
 // need four, i.e. at most 128 VGPRs.  BN254 Fr's values do not fit that.
@@ -577,6 +675,66 @@ static hipError_t launch_whole_t(int K, const BatchXform *xf, const BatchEntry *
       return hipErrorInvalidValue;
     default: return hipErrorInvalidValue;
   }
+}
+
+// One level_r8_kernel launch over `nitems` tiles of 2^tile_log elements
+// (K + each pass's log_g = tile_log), at most `grid` workgroups and at most
+// as many as are resident at once.
+template <class F, int K, bool ODD>
+static hipError_t launch_level_k(int tile_log, const BatchXform *xf, const BatchEntry *ent, uint32_t nent,
+                                 uint32_t nitems, uint32_t uniform, uint32_t *counter, int grid, hipStream_t s) {
+  constexpr int N = round_elems<F>();
+  const unsigned threads = (1u << tile_log) / N;
+  const size_t lds = (size_t)F::W * 4 << tile_log;
+  // Resident workgroups per CU for each tile size, computed once.
+  static std::once_flag once;
+  static int per_cu[13] = {};
+  static int cus = 0;
+  static hipError_t setup = hipSuccess;
+  std::call_once(once, [&] {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&level_r8_kernel<F, K, ODD>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       F::W * 4 << (std::is_same<F, FrT>::value ? 11 : 12));
+    int dev = 0;
+    if (e == hipSuccess) e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    for (int tl = 8; tl <= 12 && e == hipSuccess; tl++) {
+      const unsigned t = (1u << tl) / N;
+      if (t < 64 || t > 512) continue;
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu[tl], level_r8_kernel<F, K, ODD>, (int)t,
+                                                       (size_t)F::W * 4 << tl);
+    }
+    setup = e;
+  });
+  if (setup != hipSuccess) return setup;
+  if (tile_log < 8 || tile_log > 12 || threads < 64 || threads > 512 || per_cu[tile_log] < 1)
+    return hipErrorInvalidValue;
+  const int g = std::min<int>(std::min<int>(grid, per_cu[tile_log] * cus), (int)nitems);
+  if (g < 1) return hipSuccess;
+  hipLaunchKernelGGL((level_r8_kernel<F, K, ODD>), dim3(g), dim3(threads), lds, s, xf, ent, nent, nitems, uniform,
+                     counter);
+  return hipGetLastError();
+}
+
+// Pass lengths 5..KMAX.
+template <class F, int KMAX>
+static hipError_t launch_level_t(int K, bool odd, int tile_log, const BatchXform *xf, const BatchEntry *ent,
+                                 uint32_t nent, uint32_t nitems, uint32_t uniform, uint32_t *counter, int grid,
+                                 hipStream_t s) {
+  if (nitems == 0) return hipSuccess;
+  switch (K * 2 + (odd ? 1 : 0)) {
+#define ZNTT_LV(k)                                                                                           \
+  case 2 * k:                                                                                              \
+    if constexpr (k <= KMAX) return launch_level_k<F, k, false>(tile_log, xf, ent, nent, nitems, uniform, counter, grid, s); \
+    break;                                                                                                 \
+  case 2 * k + 1:                                                                                          \
+    if constexpr (k <= KMAX) return launch_level_k<F, k, true>(tile_log, xf, ent, nent, nitems, uniform, counter, grid, s); \
+    break;
+    ZNTT_LV(5) ZNTT_LV(6) ZNTT_LV(7) ZNTT_LV(8) ZNTT_LV(9) ZNTT_LV(10)
+#undef ZNTT_LV
+    default: break;
+  }
+  return hipErrorInvalidValue;
 }
 
 // Pass lengths the planner produces for tiles of >= 512 elements: 5..12.

@@ -24,13 +24,15 @@ struct Shape {
   int kmax;        // longest pass, in stages
   int wgs_per_cu;  // resident workgroups of the persistent grid per CU
   int order;       // transform order inside a level: 0 as given, 1 largest first
+  int r8;          // 1: passes the register-round level kernel can take run in it
   bool operator<(const Shape &o) const {
-    return std::tie(tile_log, kmax, wgs_per_cu, order) < std::tie(o.tile_log, o.kmax, o.wgs_per_cu, o.order);
+    return std::tie(tile_log, kmax, wgs_per_cu, order, r8) <
+           std::tie(o.tile_log, o.kmax, o.wgs_per_cu, o.order, o.r8);
   }
 };
 
 Shape default_shape(int field) {
-  return field == ZNTT_FIELD_BN254_FR ? Shape{11, 9, 2, 0} : Shape{12, 10, 3, 0};
+  return field == ZNTT_FIELD_BN254_FR ? Shape{11, 9, 2, 0, 1} : Shape{12, 10, 3, 0, 1};
 }
 
 }  // namespace
@@ -45,15 +47,20 @@ struct zntt_batch {
   size_t scratch_bytes = 0;
   void *dev = nullptr;                     // xf + entries + level counters
   size_t dev_bytes = 0;
-  // The launches of one execution, in order.  whole_log 0: the generic
-  // kernel over a pass level; else the whole-transform kernel for
-  // single-pass transforms of 2^whole_log elements.
+  // The launches of one execution, level by level.  Launches of one level
+  // are independent and run as parallel branches (of the captured graph).
+  enum Kind { kGeneric, kWhole, kLevelR8 };
   struct Launch {
-    int whole_log;
+    Kind kind;
+    int level;
+    int whole_log;  // kWhole: single-pass transforms of 2^whole_log elements
+    int K, odd;     // kLevelR8: pass length and parity of the first stage
     uint32_t ent_off, nent, nitems, uniform;
     const uint32_t *tw_all;  // whole launches: the one table all transforms use, else null
   };
   std::vector<Launch> launches;
+  std::vector<hipStream_t> aux;  // branch streams (launches of one level beyond the first)
+  std::vector<hipEvent_t> ev;    // fork event, then one join event per branch stream
   zntt::BatchXform *d_xf = nullptr;
   zntt::BatchEntry *d_ent = nullptr;
   uint32_t *d_counters = nullptr;
@@ -108,23 +115,50 @@ int build_plan(zntt_batch *b, const Shape &shape) {
   std::vector<zntt::BatchEntry> ent;
   b->launches.clear();
   // Transforms that fit one register-round tile go to the whole-transform
-  // kernel, one launch per size; the rest run level by level.
+  // kernel, one launch per size.  Passes with a full tile go to the
+  // register-round level kernel, one launch per (pass length, parity of the
+  // first stage); the rest to the generic level kernel.
   auto whole = [&](const zntt::BatchXform &x) {
     return x.npass == 1 && x.L >= zntt::kWholeMinLog && x.L <= zntt::whole_max_log(b->field);
   };
+  auto r8 = [&](const zntt::BatchXform &x, int p) {
+    return shape.r8 && x.K[p] + x.lg[p] == shape.tile_log && zntt::level_r8_ok(b->field, x.K[p], shape.tile_log);
+  };
   for (int p = 0; p < levels; p++) {
-    uint32_t first = 0, uniform = 0;
-    bool equal = true;
-    const uint32_t off = (uint32_t)ent.size();
+    std::map<std::pair<int, int>, std::vector<uint32_t>> groups;  // (K, odd) -> transforms
+    std::vector<uint32_t> generic;
     for (uint32_t t : idx) {
       const zntt::BatchXform &x = b->xf[t];
       if (p >= x.npass || whole(x)) continue;
-      ent.push_back(zntt::BatchEntry{t, (uint32_t)p, first, 0});
-      first += x.tiles[p];
-      if (uniform == 0) uniform = x.tiles[p];
-      equal = equal && uniform == x.tiles[p];
+      if (r8(x, p))
+        groups[{x.K[p], x.s0[p] & 1}].push_back(t);
+      else
+        generic.push_back(t);
     }
-    if (first) b->launches.push_back({0, off, (uint32_t)ent.size() - off, first, equal ? uniform : 0, nullptr});
+    // One launch over the given transforms' pass p.
+    auto add = [&](zntt_batch::Kind kind, int K, int odd, const std::vector<uint32_t> &ts) {
+      const uint32_t off = (uint32_t)ent.size();
+      uint32_t first = 0, uniform = 0;
+      bool equal = true;
+      for (uint32_t t : ts) {
+        const zntt::BatchXform &x = b->xf[t];
+        ent.push_back(zntt::BatchEntry{t, (uint32_t)p, first, 0});
+        first += x.tiles[p];
+        if (uniform == 0) uniform = x.tiles[p];
+        equal = equal && uniform == x.tiles[p];
+      }
+      b->launches.push_back({kind, p, 0, K, odd, off, (uint32_t)ent.size() - off, first, equal ? uniform : 0, nullptr});
+    };
+    // Largest launches first, so that smaller ones fill the tails.
+    std::vector<std::pair<uint64_t, std::pair<int, int>>> by_size;
+    for (auto &g : groups) {
+      uint64_t items = 0;
+      for (uint32_t t : g.second) items += b->xf[t].tiles[p];
+      by_size.push_back({items, g.first});
+    }
+    std::sort(by_size.begin(), by_size.end(), [](const auto &u, const auto &v) { return u.first > v.first; });
+    for (auto &g : by_size) add(zntt_batch::kLevelR8, g.second.first, g.second.second, groups[g.second]);
+    if (!generic.empty()) add(zntt_batch::kGeneric, 0, 0, generic);
     if (p == 0) {
       for (int L = zntt::whole_max_log(b->field); L >= zntt::kWholeMinLog; L--) {
         const uint32_t woff = (uint32_t)ent.size();
@@ -137,9 +171,28 @@ int build_plan(zntt_batch *b, const Shape &shape) {
             if (!tw) tw = b->xf[t].tw;
             same = same && tw == b->xf[t].tw;
           }
-        if (k) b->launches.push_back({L, woff, k, k, 1, same ? tw : nullptr});
+        if (k) b->launches.push_back({zntt_batch::kWhole, 0, L, 0, 0, woff, k, k, 1, same ? tw : nullptr});
       }
     }
+  }
+  // Branch streams and events for the widest level.
+  size_t width = 0;
+  for (int p = 0; p < levels; p++) {
+    size_t w = 0;
+    for (auto &l : b->launches) w += l.level == p ? 1 : 0;
+    width = std::max(width, w);
+  }
+  while (b->aux.size() + 1 < width) {
+    hipStream_t st = nullptr;
+    hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+    if (e != hipSuccess) return hip_status(e);
+    b->aux.push_back(st);
+  }
+  while (b->ev.size() < b->aux.size() + 1) {
+    hipEvent_t e0 = nullptr;
+    hipError_t e = hipEventCreateWithFlags(&e0, hipEventDisableTiming);
+    if (e != hipSuccess) return hip_status(e);
+    b->ev.push_back(e0);
   }
   const size_t xf_bytes = b->xf.size() * sizeof(zntt::BatchXform);
   const size_t it_bytes = ent.size() * sizeof(zntt::BatchEntry);
@@ -167,21 +220,41 @@ int build_plan(zntt_batch *b, const Shape &shape) {
   return ZNTT_OK;
 }
 
-// The persistent launches in order (each leaves its counters zero).
+hipError_t launch_one(zntt_batch *b, size_t i, hipStream_t s) {
+  const zntt_batch::Launch &l = b->launches[i];
+  uint32_t *ctr = b->d_counters + 2 * i;
+  const zntt::BatchEntry *ent = b->d_ent + l.ent_off;
+  switch (l.kind) {
+    case zntt_batch::kGeneric:
+      return zntt::launch_batch_level(b->field, b->d_xf, ent, l.nent, l.nitems, l.uniform, ctr, b->grid,
+                                      b->shape.tile_log, s);
+    case zntt_batch::kLevelR8:
+      return zntt::launch_batch_level_r8(b->field, l.K, l.odd != 0, b->shape.tile_log, b->d_xf, ent, l.nent,
+                                         l.nitems, l.uniform, ctr, b->grid, s);
+    default:
+      if (b->field == ZNTT_FIELD_BN254_FR) return zntt::launch_whole_fr(l.whole_log, b->d_xf, ent, l.nent, ctr, l.tw_all, s);
+      if (b->field == ZNTT_FIELD_GOLDILOCKS) return zntt::launch_whole_gl(l.whole_log, b->d_xf, ent, l.nent, ctr, l.tw_all, s);
+      return zntt::launch_whole_f31(b->field, l.whole_log, b->d_xf, ent, l.nent, ctr, l.tw_all, s);
+  }
+}
+
+// The persistent launches level by level (each leaves its counters zero).
+// A level's first launch goes on `s`, the others on branch streams that fork
+// from `s` and join it again before the next level.
 hipError_t enqueue(zntt_batch *b, hipStream_t s) {
   hipError_t e = hipSuccess;
-  for (size_t i = 0; i < b->launches.size() && e == hipSuccess; i++) {
-    const zntt_batch::Launch &l = b->launches[i];
-    if (l.whole_log == 0)
-      e = zntt::launch_batch_level(b->field, b->d_xf, b->d_ent + l.ent_off, l.nent, l.nitems, l.uniform,
-                                   b->d_counters + 2 * i, b->grid, b->shape.tile_log, s);
-    else if (b->field == ZNTT_FIELD_BN254_FR)
-      e = zntt::launch_whole_fr(l.whole_log, b->d_xf, b->d_ent + l.ent_off, l.nent, b->d_counters + 2 * i, l.tw_all, s);
-    else if (b->field == ZNTT_FIELD_GOLDILOCKS)
-      e = zntt::launch_whole_gl(l.whole_log, b->d_xf, b->d_ent + l.ent_off, l.nent, b->d_counters + 2 * i, l.tw_all, s);
-    else
-      e = zntt::launch_whole_f31(b->field, l.whole_log, b->d_xf, b->d_ent + l.ent_off, l.nent, b->d_counters + 2 * i,
-                                 l.tw_all, s);
+  for (size_t i = 0; i < b->launches.size() && e == hipSuccess;) {
+    size_t j = i;
+    while (j < b->launches.size() && b->launches[j].level == b->launches[i].level) j++;
+    if (j - i > 1) e = hipEventRecord(b->ev[0], s);
+    for (size_t k = i; k < j && e == hipSuccess; k++) {
+      hipStream_t st = k == i ? s : b->aux[k - i - 1];
+      if (k > i) e = hipStreamWaitEvent(st, b->ev[0], 0);
+      if (e == hipSuccess) e = launch_one(b, k, st);
+      if (e == hipSuccess && k > i) e = hipEventRecord(b->ev[k - i], st);
+    }
+    for (size_t k = i + 1; k < j && e == hipSuccess; k++) e = hipStreamWaitEvent(s, b->ev[k - i], 0);
+    i = j;
   }
   return e;
 }
@@ -198,6 +271,10 @@ void release(zntt_batch *b) {
   for (auto *e : b->pinned_cos) e->pins--;
   b->pinned_tw.clear();
   b->pinned_cos.clear();
+  for (auto st : b->aux) hipStreamDestroy(st);
+  for (auto e : b->ev) hipEventDestroy(e);
+  b->aux.clear();
+  b->ev.clear();
 }
 
 // Autotune decisions keyed by the batch's signature (field, sizes, directions),
@@ -344,15 +421,16 @@ int zntt_batch_autotune(zntt_batch *b, int reps, double *best_ms) {
   zntt_ctx *c = b->ctx;
   std::vector<Shape> cands;
   const int base_tl = b->field == ZNTT_FIELD_BN254_FR ? 11 : 12;
-  for (int tl : {base_tl - 1, base_tl})
-    for (int kd : {2, 3})
-      for (int w : {1, 2, 3, 4})
-        for (int order : {0, 1}) {
-          // LDS bounds the resident workgroups: 160 KiB per CU.
-          const size_t lds = (size_t)zntt::dev_words(b->field) * 4 << tl;
-          if (lds * w > 160 * 1024) continue;
-          cands.push_back(Shape{tl, tl - kd, w, order});
-        }
+  for (int r8 : {1, 0})
+    for (int tl : {base_tl - 1, base_tl})
+      for (int kd : {2, 3})
+        for (int w : {1, 2, 3, 4})
+          for (int order : {0, 1}) {
+            // LDS bounds the resident workgroups: 160 KiB per CU.
+            const size_t lds = (size_t)zntt::dev_words(b->field) * 4 << tl;
+            if (lds * w > 160 * 1024) continue;
+            cands.push_back(Shape{tl, tl - kd, w, order, r8});
+          }
   // Time on shadow copies of the data so the caller's buffers are untouched.
   std::vector<uint32_t *> real(b->xf.size());
   size_t total = 0;
@@ -418,12 +496,14 @@ int zntt_batch_autotune(zntt_batch *b, int reps, double *best_ms) {
   return ZNTT_OK;
 }
 
-int zntt_batch_shape(const zntt_batch *b, int *tile_log, int *kmax, int *wgs_per_cu, int *order) {
+int zntt_batch_shape(const zntt_batch *b, int *tile_log, int *kmax, int *wgs_per_cu, int *order,
+                     int *level_kernel) {
   if (!b) return ZNTT_INVALID_ARGUMENT;
   if (tile_log) *tile_log = b->shape.tile_log;
   if (kmax) *kmax = b->shape.kmax;
   if (wgs_per_cu) *wgs_per_cu = b->shape.wgs_per_cu;
   if (order) *order = b->shape.order;
+  if (level_kernel) *level_kernel = b->shape.r8;
   return ZNTT_OK;
 }
 

@@ -1,10 +1,11 @@
 // Serving loop for coset low-degree extensions: requests queue as they are
 // submitted, are admitted against an HBM budget and an active-request limit,
-// and run by continuous batching.  Every scheduler step is one persistent
-// launch (launch_batch_level) holding the next pass of every admitted request;
-// a request admitted while others are on the device joins the next step.  The
-// steps run in order on the server's own stream, with up to kDepth of them
-// enqueued ahead, so the device never waits for the host between steps.
+// and run by continuous batching.  Every scheduler step holds the next pass
+// of every admitted request, in a few persistent launches grouped by kernel
+// (HipDevice::classify); a request admitted while others are on the device
+// joins the next step.  The steps run in order on the server's own stream,
+// with up to kDepth of them enqueued ahead, so the device never waits for
+// the host between steps.
 //
 // The scheduling core (queue, admission, budget, steps, retirement,
 // cancellation) talks to the device through the Device interface.  HipDevice
@@ -412,13 +413,27 @@ struct HipDevice : Device {
   Slot slots[kDepth + 1];
   size_t slot_bytes = 0;
   long next_id = 0;
-  std::vector<zntt::BatchXform> xf;
-  std::vector<zntt::BatchEntry> ent;
+  // Branch streams of a step's launches beyond the first, with the step's
+  // fork event and one join event per branch stream.
+  static constexpr int kBranches = 8;
+  std::vector<hipStream_t> aux;
+  std::vector<hipEvent_t> joins;
+  hipEvent_t fork = nullptr;
 
   HipDevice(zntt_ctx *c, int f, int tl) : ctx(c), field(f), tile_log(tl) {}
 
   int init(const zntt_server_config &cfg) {
     hipError_t e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return hip_status(e);
+    e = hipEventCreateWithFlags(&fork, hipEventDisableTiming);
+    for (int k = 0; k < kBranches && e == hipSuccess; k++) {
+      hipStream_t st = nullptr;
+      hipEvent_t ev = nullptr;
+      e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+      if (e == hipSuccess) aux.push_back(st);
+      if (e == hipSuccess) e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+      if (e == hipSuccess) joins.push_back(ev);
+    }
     if (e != hipSuccess) return hip_status(e);
     e = zntt::configure_batch_lds();
     if (e != hipSuccess) return hip_status(e);
@@ -457,7 +472,11 @@ struct HipDevice : Device {
       if (sl.start) hipEventDestroy(sl.start);
       if (sl.end) hipEventDestroy(sl.end);
     }
+    for (auto st : aux) hipStreamSynchronize(st);
     if (arena_mem) hipFree(arena_mem);
+    for (auto st : aux) hipStreamDestroy(st);
+    for (auto ev : joins) hipEventDestroy(ev);
+    if (fork) hipEventDestroy(fork);
     if (stream) hipStreamDestroy(stream);
   }
 
@@ -516,41 +535,54 @@ struct HipDevice : Device {
     r->out = nullptr;
   }
 
-  // BN254 Fr requests of at least 2^kSoloLog points run each pass as a launch
-  // of its own, with the single-transform pass kernel (zero-padding its
-  // first pass): one such transform fills the device, and that kernel is
-  // faster per butterfly than the level kernel's.  The others share the
-  // step's level launch.
-  static constexpr int kSoloLog = 20;
-  bool solo(const Request *r) const {
-    if (field != ZNTT_FIELD_BN254_FR || r->L < kSoloLog) return false;
-    for (int p = 0; p < r->npass; p++) {  // the single-transform kernel's shapes
-      const int K = r->plan.K[p], tl = K + r->plan.log_g[p];
-      if (K < 5 || K > 12 || tl < 9 || tl > 11) return false;
-    }
-    return true;
-  }
+  // A step's launches.  Each item (one pass of one request) goes to one of
+  // three kernels: a request that fits one register-round tile (2^9 up to
+  // the field's largest) to the whole-transform kernel, one launch per size;
+  // a pass with a full tile to the register-round level kernel, one launch
+  // per (pass length, parity of its first stage), so the passes of all large
+  // requests that share those run in one launch; the rest (small requests)
+  // to the generic level kernel.  A step's launches are independent: they
+  // run as parallel branches of the server stream.
+  enum Kind { kGeneric, kWhole, kLevelR8 };
+  struct Group {
+    Kind kind;
+    int a, b;  // kWhole: log size; kLevelR8: pass length, parity
+    std::vector<const StepItem *> items;
+  };
+  static constexpr int kMaxGroups = 32;  // counter pairs in a step's 256-byte header
 
-  // The arguments of pass p of request r, as the level kernel derives them
-  // from the request's BatchXform.
-  static zntt::PassArgs pass_args(const Request *r, int p) {
-    zntt::PassArgs a = {};
+  static int s0_of(const Request *r, int p) {
     int s0 = 1;
     for (int q = 0; q < p; q++) s0 += r->plan.K[q];
-    a.first = p == 0;
-    a.last = p == r->npass - 1;
-    a.src = a.first ? (const uint32_t *)(r->in ? r->in : r->req.coeffs) : (const uint32_t *)r->scratch;
-    a.dst = a.last ? (uint32_t *)r->out : (uint32_t *)r->scratch;
-    a.tw = r->tw->table;
-    a.coset = a.first ? r->coset->table : nullptr;
-    a.has_scale = 0;
-    a.L = r->L;
-    a.s0 = s0;
-    a.K = r->plan.K[p];
-    a.log_g = r->plan.log_g[p];
-    a.src_len = (uint64_t)1 << r->req.log_n;
-    a.tw_one = 1;
-    return a;
+    return s0;
+  }
+
+  std::vector<Group> classify(const std::vector<StepItem> &items) const {
+    std::vector<Group> g;
+    auto find = [&](Kind k, int a, int b) -> Group & {
+      for (auto &x : g)
+        if (x.kind == k && x.a == a && x.b == b) return x;
+      g.push_back(Group{k, a, b, {}});
+      return g.back();
+    };
+    for (const auto &it : items) {
+      const Request *r = it.r;
+      const int K = r->plan.K[it.pass], tl = K + r->plan.log_g[it.pass];
+      if (r->npass == 1 && r->L >= zntt::kWholeMinLog && r->L <= zntt::whole_max_log(field))
+        find(kWhole, r->L, 0).items.push_back(&it);
+      else if (tl == tile_log && zntt::level_r8_ok(field, K, tl))
+        find(kLevelR8, K, s0_of(r, it.pass) & 1).items.push_back(&it);
+      else
+        find(kGeneric, 0, 0).items.push_back(&it);
+    }
+    // Largest launches first, so that the smaller ones fill their tails.
+    auto work = [](const Group &x) {
+      uint64_t w = 0;
+      for (const StepItem *it : x.items) w += (uint64_t)1 << it->r->L;
+      return w;
+    };
+    std::stable_sort(g.begin(), g.end(), [&](const Group &u, const Group &v) { return work(u) > work(v); });
+    return g;
   }
 
   long submit_step(const std::vector<StepItem> &items) override {
@@ -561,54 +593,80 @@ struct HipDevice : Device {
         break;
       }
     if (!sl) return -ZNTT_DEVICE_ERROR;  // more steps in flight than slots: a scheduler bug
-    size_t n = 0;  // items of the level launch
-    for (const auto &it : items) n += solo(it.r) ? 0 : 1;
+    const std::vector<Group> groups = classify(items);
+    if ((int)groups.size() > kMaxGroups) return -ZNTT_DEVICE_ERROR;
+    const size_t n = items.size();
     uint8_t *h = (uint8_t *)sl->host;
-    memset(h, 0, 256);  // the step's counters (next item, finished workgroups) start at zero
+    memset(h, 0, 256);  // the launches' counters (next item, finished workgroups) start at zero
     auto *hx = (zntt::BatchXform *)(h + 256);
     auto *he = (zntt::BatchEntry *)(h + 256 + align_up(sizeof(zntt::BatchXform) * n, 256));
-    uint32_t first = 0, uniform = 0;
+    struct L {
+      size_t ent_off, nent;
+      uint32_t nitems, uniform;
+    };
+    std::vector<L> ls;
     size_t i = 0;
-    for (const auto &item : items) {
-      const Request *r = item.r;
-      if (solo(r)) continue;
-      zntt::BatchXform &x = hx[i];
-      memset(&x, 0, sizeof x);
-      x.src = (const uint32_t *)(r->in ? r->in : r->req.coeffs);
-      x.src_len = (uint64_t)1 << r->req.log_n;
-      x.data = (uint32_t *)r->out;
-      x.scratch = (uint32_t *)r->scratch;
-      x.tw = r->tw->table;
-      x.coset = r->coset->table;
-      x.L = r->L;
-      x.npass = r->npass;
-      int s0 = 1;
-      for (int p = 0; p < r->npass; p++) {
-        x.K[p] = r->plan.K[p];
-        x.lg[p] = r->plan.log_g[p];
-        x.s0[p] = s0;
-        x.tiles[p] = (uint32_t)(((size_t)1 << r->L) >> (r->plan.K[p] + r->plan.log_g[p]));
-        s0 += r->plan.K[p];
+    for (const Group &g : groups) {
+      const size_t off = i;
+      uint32_t first = 0, uniform = 0;
+      for (const StepItem *item : g.items) {
+        const Request *r = item->r;
+        zntt::BatchXform &x = hx[i];
+        memset(&x, 0, sizeof x);
+        x.src = (const uint32_t *)(r->in ? r->in : r->req.coeffs);
+        x.src_len = (uint64_t)1 << r->req.log_n;
+        x.data = (uint32_t *)r->out;
+        x.scratch = (uint32_t *)r->scratch;
+        x.tw = r->tw->table;
+        x.coset = r->coset->table;
+        x.L = r->L;
+        x.npass = r->npass;
+        int s0 = 1;
+        for (int p = 0; p < r->npass; p++) {
+          x.K[p] = r->plan.K[p];
+          x.lg[p] = r->plan.log_g[p];
+          x.s0[p] = s0;
+          x.tiles[p] = (uint32_t)(((size_t)1 << r->L) >> (r->plan.K[p] + r->plan.log_g[p]));
+          s0 += r->plan.K[p];
+        }
+        const uint32_t tiles = g.kind == kWhole ? 1u : x.tiles[item->pass];
+        he[i] = zntt::BatchEntry{(uint32_t)i, (uint32_t)item->pass, first, 0};
+        if (i == off) uniform = tiles;
+        if (tiles != uniform) uniform = 0;
+        first += tiles;
+        i++;
       }
-      he[i] = zntt::BatchEntry{(uint32_t)i, (uint32_t)item.pass, first, 0};
-      const uint32_t tiles = x.tiles[item.pass];
-      if (i == 0) uniform = tiles;
-      if (tiles != uniform) uniform = 0;
-      first += tiles;
-      i++;
+      ls.push_back(L{off, i - off, first, uniform});
     }
     const size_t bytes = (uint8_t *)(he + n) - h;
     uint8_t *d = (uint8_t *)sl->dev;
+    const auto *dx = (const zntt::BatchXform *)(d + 256);
+    const auto *de = (const zntt::BatchEntry *)(d + ((uint8_t *)he - h));
     hipError_t e = hipEventRecord(sl->start, stream);
-    if (n > 0) {
-      if (e == hipSuccess) e = hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, stream);
-      if (e == hipSuccess)
-        e = zntt::launch_batch_level(field, (const zntt::BatchXform *)(d + 256),
-                                     (const zntt::BatchEntry *)(d + ((uint8_t *)he - h)), (uint32_t)n, first,
-                                     uniform, (uint32_t *)d, grid, tile_log, stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, stream);
+    if (groups.size() > 1 && e == hipSuccess) e = hipEventRecord(fork, stream);
+    for (size_t k = 0; k < groups.size() && e == hipSuccess; k++) {
+      const Group &g = groups[k];
+      const L &l = ls[k];
+      hipStream_t st = k == 0 ? stream : aux[(k - 1) % aux.size()];
+      if (k > 0) e = hipStreamWaitEvent(st, fork, 0);
+      uint32_t *ctr = (uint32_t *)d + 2 * k;
+      if (e != hipSuccess) break;
+      if (g.kind == kGeneric)
+        e = zntt::launch_batch_level(field, dx, de + l.ent_off, (uint32_t)l.nent, l.nitems, l.uniform, ctr, grid,
+                                     tile_log, st);
+      else if (g.kind == kLevelR8)
+        e = zntt::launch_batch_level_r8(field, g.a, g.b != 0, tile_log, dx, de + l.ent_off, (uint32_t)l.nent,
+                                        l.nitems, l.uniform, ctr, grid, st);
+      else if (field == ZNTT_FIELD_BN254_FR)
+        e = zntt::launch_whole_fr(g.a, dx, de + l.ent_off, (uint32_t)l.nent, ctr, nullptr, st);
+      else if (field == ZNTT_FIELD_GOLDILOCKS)
+        e = zntt::launch_whole_gl(g.a, dx, de + l.ent_off, (uint32_t)l.nent, ctr, nullptr, st);
+      else
+        e = zntt::launch_whole_f31(field, g.a, dx, de + l.ent_off, (uint32_t)l.nent, ctr, nullptr, st);
+      if (e == hipSuccess && k > 0) e = hipEventRecord(joins[(k - 1) % aux.size()], st);
+      if (e == hipSuccess && k > 0) e = hipStreamWaitEvent(stream, joins[(k - 1) % aux.size()], 0);
     }
-    for (const auto &item : items)
-      if (e == hipSuccess && solo(item.r)) e = zntt::launch_pass_fr_padded(pass_args(item.r, item.pass), stream);
     if (e == hipSuccess) e = hipEventRecord(sl->end, stream);
     if (e != hipSuccess) return -hip_status(e);
     sl->id = next_id++;

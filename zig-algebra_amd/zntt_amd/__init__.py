@@ -220,7 +220,7 @@ def _load():
     L.zntt_batch_status.argtypes = [vp, vp]
     L.zntt_batch_autotune.argtypes = [vp, i32, ctypes.POINTER(ctypes.c_double)]
     ip = ctypes.POINTER(ctypes.c_int)
-    L.zntt_batch_shape.argtypes = [vp, ip, ip, ip, ip]
+    L.zntt_batch_shape.argtypes = [vp, ip, ip, ip, ip, ip]
     L.zntt_batch_destroy.argtypes = [vp]
     L.zntt_batch_destroy.restype = None
     L.zntt_server_create.argtypes = [vp, ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)]
@@ -653,9 +653,9 @@ class Batch:
         return ms.value
 
     def shape(self) -> dict:
-        v = [ctypes.c_int() for _ in range(4)]
+        v = [ctypes.c_int() for _ in range(5)]
         check(lib.zntt_batch_shape(self._h, *[ctypes.byref(x) for x in v]))
-        return dict(zip(("tile_log", "kmax", "wgs_per_cu", "order"), [x.value for x in v]))
+        return dict(zip(("tile_log", "kmax", "wgs_per_cu", "order", "level_kernel"), [x.value for x in v]))
 
     def close(self) -> None:
         if self._h:
