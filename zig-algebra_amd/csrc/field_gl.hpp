@@ -16,19 +16,26 @@ constexpr uint64_t kGlEps = 0xFFFFFFFFull;  // 2^64 mod p
 
 __device__ __forceinline__ uint64_t gl_canon(uint64_t x) { return x >= kGlP ? x - kGlP : x; }
 
-// (hi * 2^64 + lo) mod p, canonical.
+// (hi * 2^64 + lo) mod p, canonical, for any lo, hi.  With hi = h1 * 2^32 + h0,
+// x = lo + h0 * 2^64 + h1 * 2^96 = lo + h0 * eps - h1.  The term
+// z = (h0 + 1) * eps - h1 = h0 * eps + (eps - h1) is one multiply-add whose
+// 32-bit addend is ~h1, never negative and below 2^64 - 2^32, and
+// x = lo + z - eps.  r = lo + z (mod 2^64):
+// * carried: lo + z = r + 2^64 = r + eps, so x = r, and r < z < p;
+// * otherwise x = r - eps, in [0, p) when r >= eps, else r - eps + p, which
+//   is r - 2 eps mod 2^64.
+// One carry test and one compare against eps pick the multiple of eps to
+// subtract; there is no separate canonicalising step.
 __device__ __forceinline__ uint64_t gl_reduce128(uint64_t lo, uint64_t hi) {
   const uint32_t h0 = (uint32_t)hi, h1 = (uint32_t)(hi >> 32);
-  // lo - h1 * 2^96-part: 2^96 = -1
-  uint64_t t0 = lo - h1;
-  if (lo < (uint64_t)h1) t0 -= kGlEps;  // borrowed 2^64 = 2^32 - 1
-  // h0 * 2^64 = h0 * (2^32 - 1)
-  const uint64_t t1 = ((uint64_t)h0 << 32) - h0;
-  uint64_t r = t0 + t1;
-  if (r < t1) r += kGlEps;  // carried 2^64; cannot carry again (r < t1 <= 2^64 - 2^33 + 1)
-  return gl_canon(r);
+  const uint64_t z = (uint64_t)h0 * kGlEps + (uint32_t)~h1;
+  const uint64_t r = lo + z;
+  const bool carried = r < z;
+  const uint64_t k = carried ? 0 : (r < kGlEps ? 2 * kGlEps : kGlEps);
+  return r - k;
 }
 
+// a * b mod p, canonical; a, b any u64 (the 128-bit product is exact).
 __device__ __forceinline__ uint64_t gl_mul(uint64_t a, uint64_t b) {
   const uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32);
   const uint32_t b0 = (uint32_t)b, b1 = (uint32_t)(b >> 32);
@@ -38,37 +45,6 @@ __device__ __forceinline__ uint64_t gl_mul(uint64_t a, uint64_t b) {
   const uint64_t hi = (uint64_t)a1 * b1 + (t >> 32) + (u >> 32);
   const uint64_t lo = (u << 32) | (uint32_t)p00;
   return gl_reduce128(lo, hi);
-}
-
-// Lazy forms for the transform's inner loop.  Values there are only "weakly
-// reduced": any u64, congruent to the true value mod p; stores canonicalise.
-//
-// (hi * 2^64 + lo) mod p up to one multiple of p, for any lo, hi: with
-// hi = h1 * 2^32 + h0, x = lo + h0 * 2^64 + h1 * 2^96 = lo + h0 * eps - h1.
-// lo - h1 borrows only when lo < h1, and then the wrapped value exceeds eps,
-// so subtracting eps (= adding back 2^64 - p) cannot borrow again; t0 + t1
-// with t1 = h0 * eps <= (2^32 - 1)^2 carries to a value below t1, so adding
-// eps cannot carry again.
-__device__ __forceinline__ uint64_t gl_reduce_lazy(uint64_t lo, uint64_t hi) {
-  const uint32_t h0 = (uint32_t)hi, h1 = (uint32_t)(hi >> 32);
-  uint64_t t0 = lo - h1;
-  t0 -= (lo < (uint64_t)h1) ? kGlEps : 0;
-  const uint64_t t1 = (uint64_t)h0 * kGlEps;
-  uint64_t r = t0 + t1;
-  r += (r < t1) ? kGlEps : 0;
-  return r;
-}
-
-// a * b mod p, weakly reduced; a, b any u64.
-__device__ __forceinline__ uint64_t gl_mul_lazy(uint64_t a, uint64_t b) {
-  const uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32);
-  const uint32_t b0 = (uint32_t)b, b1 = (uint32_t)(b >> 32);
-  const uint64_t p00 = (uint64_t)a0 * b0;
-  const uint64_t t = (uint64_t)a0 * b1 + (p00 >> 32);
-  const uint64_t u = (uint64_t)a1 * b0 + (uint32_t)t;
-  const uint64_t hi = (uint64_t)a1 * b1 + (t >> 32) + (u >> 32);
-  const uint64_t lo = (u << 32) | (uint32_t)p00;
-  return gl_reduce_lazy(lo, hi);
 }
 
 // a, b canonical.
@@ -86,12 +62,15 @@ __device__ __forceinline__ uint64_t gl_sub(uint64_t a, uint64_t b) {
 
 __device__ __forceinline__ uint64_t gl_neg(uint64_t a) { return a == 0 ? 0 : kGlP - a; }
 
+// Values in the transform's inner loop are only "weakly reduced": any u64,
+// congruent to the true value mod p; stores canonicalise.
+
 // Radix-2 DIT butterfly on weakly reduced values: (u, v) <- (u + w v, u - w v),
 // outputs weakly reduced.  With t = w v reduced below p, u + t carries to a
 // value below t (< p), so adding eps cannot carry again; u - t borrows to a
 // value above 2^64 - p = eps, so subtracting eps cannot borrow again.
 __device__ __forceinline__ void gl_butterfly(uint64_t &u, uint64_t &v, uint64_t w) {
-  const uint64_t t = gl_canon(gl_mul_lazy(v, w));
+  const uint64_t t = gl_mul(v, w);
   uint64_t s = u + t;
   s += (s < t) ? kGlEps : 0;
   uint64_t d = u - t;
