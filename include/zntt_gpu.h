@@ -109,8 +109,31 @@ void zntt_ctx_destroy(zntt_ctx *ctx);
 uint64_t zntt_ctx_bytes_in_use(const zntt_ctx *ctx);
 /* Frees every cached twiddle / coset table that no live batch or server
  * uses (those stay, pinned), and the context's scratch and staging buffers.
- * Waits for the context's stream and the device first. */
+ * A reservation (zntt_ctx_reserve) stays.  Waits for the context's stream
+ * and the device first. */
 int zntt_ctx_trim(zntt_ctx *ctx);
+/* Confines device-memory failures of the seven zig-ntt functions below to
+ * this call, for sizes up to 2^max_log_n of every field.  zig-ntt's error
+ * sets have no OutOfMemory for ntt, intt, bitReverse or *WithTwiddles
+ * (root.zig:33-212); a backend that allocates device memory lazily could
+ * only turn a full device into a panic there.  This call allocates, up
+ * front, everything those functions use at such sizes: the staging and
+ * pass-scratch buffers, a foreign table's upload buffer, and two table
+ * slots of 2^max_log_n entries.  Afterwards ntt, intt, bitReverse,
+ * nttWithTwiddles and inttWithTwiddles at log_n <= max_log_n allocate
+ * nothing: a table the cache does not hold is built into the least recently
+ * used slot (a root's powers for ntt / intt, the derived inverse of a
+ * registered table for inttWithTwiddles).  precomputeTwiddles still
+ * allocates its table (its error set has OutOfMemory).  The reservation
+ * lasts until the context is freed; a second call with a larger max_log_n
+ * grows it, a smaller one changes nothing.  Returns ZNTT_OUT_OF_MEMORY or
+ * ZNTT_DEVICE_ERROR (the reservation is then unchanged; buffers it already
+ * grew stay), ZNTT_UNSUPPORTED above ZNTT_MAX_LOG_N.  No zig-ntt
+ * counterpart: the shim calls it once after zntt_ctx_create.
+ * Device bytes, sized for BN254 Fr (the largest element: 32 B in the
+ * caller's layout, 36 B on the device): 2^max_log_n x 244 B, 4.1 GB at
+ * 2^24. */
+int zntt_ctx_reserve(zntt_ctx *ctx, size_t max_log_n);
 const char *zntt_status_string(int status);
 /* Bytes per element of a field, 0 for an unknown id. */
 size_t zntt_element_size(int field);

@@ -295,6 +295,96 @@ void streams_share_scratch_in_order() {
   zntt_ctx_destroy(c);
 }
 
+// After zntt_ctx_reserve(ctx, 12) the seven zig-ntt functions at sizes up
+// to 2^12 make no allocation or free and hold no new device bytes, for
+// every field, every root (tables they do not find are built into the
+// reservation's slots), registered and foreign twiddle tables, and across a
+// trim.  Above the reservation they allocate as before.
+void reserved_calls_allocate_nothing() {
+  zntt_ctx *c = nullptr;
+  CHECK(zntt_ctx_create(&c, 0, 1ull << 30));
+  CHECK(zntt_ctx_reserve(c, 12));
+  const int fields[4] = {ZNTT_FIELD_BN254_FR, ZNTT_FIELD_GOLDILOCKS, ZNTT_FIELD_BABYBEAR, ZNTT_FIELD_KOALABEAR};
+  const size_t logs[4] = {1, 5, 11, 12};
+  // Registered tables (precomputeTwiddles allocates them: its error set has
+  // OutOfMemory), one per field at 2^12.
+  std::vector<std::vector<std::vector<uint8_t>>> reg(4);
+  std::vector<std::vector<void *>> reg_ptrs(4);
+  for (int fi = 0; fi < 4; fi++) {
+    const size_t eb = zntt_element_size(fields[fi]);
+    uint8_t root[32] = {0};
+    CHECK(zntt_primitive_root(fields[fi], 12, root));
+    for (size_t s = 0; s < 12; s++) {
+      reg[fi].emplace_back(((size_t)1 << s) * eb, 0);
+      reg_ptrs[fi].push_back(reg[fi].back().data());
+    }
+    CHECK(zntt_precompute_twiddles(c, fields[fi], 12, root, reg_ptrs[fi].data(), 12));
+  }
+  long before = fake_hip_alloc_calls();
+  uint64_t bytes = zntt_ctx_bytes_in_use(c);
+  for (int rep = 0; rep < 2; rep++) {
+    for (int fi = 0; fi < 4; fi++) {
+      const int f = fields[fi];
+      const size_t eb = zntt_element_size(f);
+      for (size_t L : logs) {
+        const size_t n = (size_t)1 << L;
+        std::vector<uint8_t> data(n * eb, 1);
+        uint8_t root[32] = {0}, other[32] = {0};
+        CHECK(zntt_primitive_root(f, L, root));
+        other[0] = (uint8_t)(3 + L + rep);  // a root no cache holds
+        CHECK(zntt_ntt(c, f, data.data(), n, L, root));
+        CHECK(zntt_intt(c, f, data.data(), n, L, root));
+        CHECK(zntt_ntt(c, f, data.data(), n, L, other));
+        CHECK(zntt_intt(c, f, data.data(), n, L, other));
+        CHECK(zntt_bit_reverse(c, f, data.data(), n));
+        // A foreign table (never registered), both directions.
+        std::vector<std::vector<uint8_t>> st;
+        std::vector<const void *> sp;
+        std::vector<size_t> lens;
+        for (size_t s = 0; s < L; s++) {
+          st.emplace_back(((size_t)1 << s) * eb, 2);
+          sp.push_back(st.back().data());
+          lens.push_back((size_t)1 << s);
+        }
+        CHECK(zntt_ntt_with_twiddles(c, f, data.data(), n, L, sp.data(), lens.data(), L));
+        CHECK(zntt_intt_with_twiddles(c, f, data.data(), n, L, sp.data(), lens.data(), L));
+      }
+      // The registered table, both directions (the inverse is derived into a slot).
+      const size_t n = (size_t)1 << 12;
+      std::vector<uint8_t> data(n * eb, 1);
+      std::vector<size_t> lens;
+      for (size_t s = 0; s < 12; s++) lens.push_back((size_t)1 << s);
+      const void *const *sp = (const void *const *)reg_ptrs[fi].data();
+      CHECK(zntt_ntt_with_twiddles(c, f, data.data(), n, 12, sp, lens.data(), 12));
+      CHECK(zntt_intt_with_twiddles(c, f, data.data(), n, 12, sp, lens.data(), 12));
+    }
+    EXPECT(fake_hip_alloc_calls() == before);
+    EXPECT(zntt_ctx_bytes_in_use(c) == bytes);
+    if (rep == 0) {
+      // A trim frees the cached and registered tables (counted frees), and
+      // the reservation outlives it: the second round allocates nothing
+      // either (registered tables now go the foreign way).
+      CHECK(zntt_ctx_trim(c));
+      before = fake_hip_alloc_calls();
+      bytes = zntt_ctx_bytes_in_use(c);
+    }
+  }
+  // Above the reservation the calls allocate (and can run out of memory).
+  std::vector<uint8_t> big(((size_t)1 << 13) * 32, 1);
+  uint8_t r13[32] = {0};
+  CHECK(zntt_primitive_root(F, 13, r13));
+  CHECK(zntt_ntt(c, F, big.data(), (size_t)1 << 13, 13, r13));
+  EXPECT(fake_hip_alloc_calls() > before);
+  // A larger reservation grows it; a smaller one changes nothing.
+  const uint64_t b12 = zntt_ctx_bytes_in_use(c);
+  CHECK(zntt_ctx_reserve(c, 10));
+  EXPECT(zntt_ctx_bytes_in_use(c) == b12);
+  EXPECT(zntt_ctx_reserve(c, ZNTT_MAX_LOG_N + 1) == ZNTT_UNSUPPORTED);
+  for (int fi = 0; fi < 4; fi++)
+    CHECK(zntt_free_twiddles(c, (const void *const *)reg_ptrs[fi].data(), 12));
+  zntt_ctx_destroy(c);
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
@@ -307,6 +397,7 @@ int main(int argc, char **argv) {
   else if (s == "late_destroy_after_new_ctx") late_destroy_after_new_ctx();
   else if (s == "noalloc_allocates_nothing") noalloc_allocates_nothing();
   else if (s == "streams_share_scratch_in_order") streams_share_scratch_in_order();
+  else if (s == "reserved_calls_allocate_nothing") reserved_calls_allocate_nothing();
   else return 1;
   printf("ok %s\n", s.c_str());
   return 0;

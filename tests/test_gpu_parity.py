@@ -376,6 +376,59 @@ def test_context_budget_exhaustion(Z):
         small.close()
 
 
+@pytest.mark.parametrize("field", FIELDS, ids=IDS)
+def test_reserved_context_matches_oracle_and_holds_no_new_memory(Z, field):
+    """After Context.reserve(14) (zntt_ctx_reserve) the seven zig-ntt
+    functions at sizes up to 2^14 -- primitive and other roots, bit reversal,
+    foreign and registered twiddle tables in both directions -- match the
+    oracle bit for bit and leave the context's device bytes unchanged: the
+    tables they need are built into the reservation's slots.  The budget
+    holds the reservation and one registered table, nothing more."""
+    say("reserved", FNAME[field])
+    c = Z.Context(0, hbm_budget_bytes=8 << 20)
+    try:
+        c.reserve(14)
+        root14 = O.primitive_root(field, 14)[1]
+        reg = c.precompute_twiddles(field, 14, root14)
+        st, otw = O.precompute_twiddles(field, 14, root14)
+        assert st == O.OK
+        held = c.bytes_in_use()
+        other = O.from_int(field, 123456789)
+        for log_n in (1, 7, 12, 14):
+            n = 1 << log_n
+            x = O.gen_uniform(field, 4000 + log_n, n)
+            for root in (O.primitive_root(field, log_n)[1], other):
+                for fwd in (True, False):
+                    exp = x.copy()
+                    assert (O.ntt if fwd else O.intt)(field, exp, log_n, root) == O.OK
+                    got = x.copy()
+                    (c.ntt if fwd else c.intt)(field, got, log_n, root)
+                    assert np.array_equal(got, exp), (log_n, fwd)
+            exp = x.copy()
+            O.bit_reverse(field, exp)
+            got = x.copy()
+            c.bit_reverse(field, got)
+            assert np.array_equal(got, exp)
+            stages = [O.gen_uniform(field, 4100 + s, 1 << s) for s in range(log_n)]
+            for inverse in (False, True):
+                exp = x.copy()
+                O.ntt_with_twiddles(field, exp, log_n, stages, inverse=inverse)
+                got = x.copy()
+                (c.intt_with_twiddles if inverse else c.ntt_with_twiddles)(field, got, log_n, stages)
+                assert np.array_equal(got, exp), ("foreign", log_n, inverse)
+        x = O.gen_uniform(field, 4200, 1 << 14)
+        for inverse in (False, True):
+            exp = x.copy()
+            O.ntt_with_twiddles(field, exp, 14, otw, inverse=inverse)
+            got = x.copy()
+            (c.intt_with_twiddles if inverse else c.ntt_with_twiddles)(field, got, 14, reg)
+            assert np.array_equal(got, exp), ("registered", inverse)
+        assert c.bytes_in_use() == held
+        reg.free()
+    finally:
+        c.close()
+
+
 # ---------------------------------------------------------------------------
 # Boundary: device-resident twiddled calls, stale registrations, streams
 # ---------------------------------------------------------------------------

@@ -21,7 +21,10 @@ Two more scenarios use the stand-in runtime's call counters:
 * zntt_execute_device_noalloc makes no allocation or free once its tables
   are cached, and refuses oversized or unprepared transforms up front;
 * device transforms on two streams of one context wait for each other's use
-  of the shared scratch exactly when the stream changes.
+  of the shared scratch exactly when the stream changes;
+* after zntt_ctx_reserve the seven zig-ntt functions allocate nothing at the
+  reserved sizes (every field, new roots, registered and foreign tables,
+  across a trim), so device memory can run out only at the reservation.
 """
 import os
 import shutil
@@ -33,7 +36,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 SIM = os.path.join(HERE, "hostsim")
 OUT = os.path.join(SIM, "_build")
 SCENARIOS = ["batch_outlives_ctx", "trim_with_live_batch", "server_outlives_ctx", "trim_with_live_server",
-             "late_destroy_after_new_ctx", "noalloc_allocates_nothing", "streams_share_scratch_in_order"]
+             "late_destroy_after_new_ctx", "noalloc_allocates_nothing", "streams_share_scratch_in_order",
+             "reserved_calls_allocate_nothing"]
 
 
 @pytest.fixture(scope="module")

@@ -33,8 +33,23 @@ struct TwiddleEntry {
   size_t inv_bytes = 0;
   std::vector<const void *> key;     // host stage pointers of a registered table
   uint64_t fingerprint = 0;          // of sampled entries of a registered table's host stages
+  uint64_t reg_id = 0;               // a registered table's number, unique in its context
   uint64_t last_use = 0;
   int pins = 0;                      // live batch plans reading this table
+};
+
+// A table slot of a reservation (zntt_ctx_reserve): device storage for one
+// table of up to 2^resv_log_n entries of any field.  The reserved calls build
+// the tables they need into the least recently used slot instead of
+// allocating; nothing else reads the slots, so no batch or server pins them.
+struct ResvSlot {
+  uint32_t *table = nullptr;
+  size_t bytes = 0;
+  int field = -1;                                // -1: holds no table
+  int L = 0;
+  std::vector<zntt_detail::Elem> chain;          // a table of powers: chain[k] = root^(2^k)
+  uint64_t inv_of = 0;                           // nonzero: inverse of the registered table reg_id
+  uint64_t last_use = 0;
 };
 
 struct CosetEntry {
@@ -76,6 +91,14 @@ struct zntt_ctx {
   size_t m31_table_bytes = 0;
   uint64_t m31_root = 0;
   int m31_log_n = -1;
+  // Reservation (zntt_ctx_reserve): the seven zig-ntt entry points at sizes
+  // up to 2^resv_log_n run in the reusable buffers above, pre-sized, and in
+  // these slots, and allocate nothing.  -1: no reservation.
+  int resv_log_n = -1;
+  ResvSlot resv[2];
+  uint32_t *resv_consts = nullptr;  // a table build's root chain upload
+  size_t resv_consts_bytes = 0;
+  uint64_t next_reg_id = 1;
   // Ordering of the shared buffers' users across streams (CallGuard).
   hipEvent_t shared_done = nullptr;
   hipStream_t shared_stream = nullptr;

@@ -157,12 +157,23 @@ int dev_alloc(zntt_ctx *c, size_t bytes, void **out) {
   return ZNTT_OK;
 }
 
-// Grows a reusable buffer to at least `bytes`.
+// Grows a reusable buffer to at least `bytes`.  On a context with a
+// reservation the new buffer is allocated before the old one is freed, and a
+// failed growth keeps the old buffer, so the reserved sizes stay covered.
 int ensure(zntt_ctx *c, void **buf, size_t *have, size_t bytes) {
   if (*have >= bytes) return ZNTT_OK;
   if (*buf) {
     hipStreamSynchronize(c->stream);
     hipDeviceSynchronize();
+    if (c->resv_log_n >= 0) {
+      void *grown = nullptr;
+      int st = dev_alloc(c, bytes, &grown);
+      if (st) return st;
+      dev_free(c, *buf, *have);
+      *buf = grown;
+      *have = bytes;
+      return ZNTT_OK;
+    }
     dev_free(c, *buf, *have);
     *buf = nullptr;
     *have = 0;
@@ -193,7 +204,8 @@ int ensure_pinned(zntt_ctx *c, size_t bytes) {
 
 // Frees every cache entry no batch or server has pinned (every entry with
 // `all`), and the context's reusable buffers.  Pinned entries stay: live
-// children read them.  Caller holds the context lock.
+// children read them.  A reservation stays too (its buffers and slots), unless
+// `all`.  Caller holds the context lock.
 void release_cache(zntt_ctx *c, bool all = false) {
   hipStreamSynchronize(c->stream);
   hipDeviceSynchronize();
@@ -214,12 +226,26 @@ void release_cache(zntt_ctx *c, bool all = false) {
     dev_free(c, it->table, it->bytes);
     it = c->cosets.erase(it);
   }
-  dev_free(c, c->scratch, c->scratch_bytes);
-  dev_free(c, c->staging, c->staging_bytes);
-  dev_free(c, c->aux, c->aux_bytes);
+  if (all || c->resv_log_n < 0) {
+    dev_free(c, c->scratch, c->scratch_bytes);
+    dev_free(c, c->staging, c->staging_bytes);
+    dev_free(c, c->aux, c->aux_bytes);
+    c->scratch = c->staging = c->aux = nullptr;
+    c->scratch_bytes = c->staging_bytes = c->aux_bytes = 0;
+  }
+  if (all) {
+    for (ResvSlot &sl : c->resv) {
+      dev_free(c, sl.table, sl.bytes);
+      sl = ResvSlot();
+    }
+    dev_free(c, c->resv_consts, c->resv_consts_bytes);
+    c->resv_consts = nullptr;
+    c->resv_consts_bytes = 0;
+    c->resv_log_n = -1;
+  }
   dev_free(c, c->m31_table, c->m31_table_bytes);
-  c->scratch = c->staging = c->aux = c->m31_table = nullptr;
-  c->scratch_bytes = c->staging_bytes = c->aux_bytes = c->m31_table_bytes = 0;
+  c->m31_table = nullptr;
+  c->m31_table_bytes = 0;
   c->m31_log_n = -1;
   if (c->pinned) hipHostFree(c->pinned);
   c->pinned = nullptr;
@@ -257,6 +283,34 @@ int upload_small(zntt_ctx *c, const std::vector<uint32_t> &words, hipStream_t s,
   return hip_status(e);
 }
 
+// The constants a table build reads, in the dev layout: the root chain's
+// first max(L, 1) entries, then one.
+std::vector<uint32_t> chain_consts(int field, const std::vector<Elem> &chain, int L) {
+  std::vector<uint32_t> consts;
+  const int W = zntt::dev_words(field);
+  for (int k = 0; k < std::max(L, 1); k++) {
+    auto d = f_to_dev(field, chain[k]);
+    consts.insert(consts.end(), d.begin(), d.begin() + W);
+  }
+  auto one = f_to_dev(field, f_one(field));
+  consts.insert(consts.end(), one.begin(), one.begin() + W);
+  return consts;
+}
+
+// Builds the table of powers of a root (2^L entries, see power_table) at
+// `table` from its chain constants, uploaded through `cbuf` (consts.size()
+// words of device memory), and waits for it.
+hipError_t build_powers(int field, int L, const std::vector<uint32_t> &consts, uint32_t *table, uint32_t *cbuf,
+                        hipStream_t s) {
+  hipError_t he = hipMemcpyAsync(cbuf, consts.data(), consts.size() * 4, hipMemcpyHostToDevice, s);
+  const uint32_t *one_dev = cbuf + (size_t)std::max(L, 1) * zntt::dev_words(field);
+  if (he == hipSuccess) he = zntt::launch_build_table(field, L, table, cbuf, one_dev, s);
+  if (he == hipSuccess) he = hipStreamSynchronize(s);  // `consts` is the caller's
+  return he;
+}
+
+TwiddleEntry *find_power_table(zntt_ctx *c, int field, const Elem &root, int L);
+
 // Device table for the DIT DAG of ntt(root) at size 2^L.  A table built for
 // (root', L') with L' >= L and root'^(2^(L'-L)) == root serves the query: stage
 // s of both tables is {root'^(j 2^(L'-s))}, so the smaller table is a prefix.
@@ -276,24 +330,14 @@ int power_table(zntt_ctx *c, int field, const Elem &root, int L, hipStream_t s, 
   e.bytes = ((size_t)1 << L) * dev_bytes(field);
   int st = dev_alloc(c, e.bytes, (void **)&e.table);
   if (st) return st;
-  std::vector<uint32_t> consts;
-  const int W = zntt::dev_words(field);
-  for (int k = 0; k < std::max(L, 1); k++) {
-    auto d = f_to_dev(field, e.chain[k]);
-    consts.insert(consts.end(), d.begin(), d.begin() + W);
-  }
-  auto one = f_to_dev(field, f_one(field));
-  consts.insert(consts.end(), one.begin(), one.begin() + W);
+  const std::vector<uint32_t> consts = chain_consts(field, e.chain, L);
   void *cbuf = nullptr;
-  st = upload_small(c, consts, s, &cbuf);
+  st = dev_alloc(c, consts.size() * 4, &cbuf);
   if (st) {
     dev_free(c, e.table, e.bytes);
     return st;
   }
-  const uint32_t *pows = (const uint32_t *)cbuf;
-  const uint32_t *one_dev = pows + (size_t)std::max(L, 1) * W;
-  hipError_t he = zntt::launch_build_table(field, L, e.table, pows, one_dev, s);
-  if (he == hipSuccess) he = hipStreamSynchronize(s);
+  hipError_t he = build_powers(field, L, consts, e.table, (uint32_t *)cbuf, s);
   dev_free(c, cbuf, consts.size() * 4);
   if (he != hipSuccess) {
     dev_free(c, e.table, e.bytes);
@@ -302,6 +346,51 @@ int power_table(zntt_ctx *c, int field, const Elem &root, int L, hipStream_t s, 
   e.last_use = ++c->tick;
   c->tables.push_back(std::move(e));
   *out = &c->tables.back();
+  return ZNTT_OK;
+}
+
+// The reservation's slot to rebuild: an empty one, else the least recently
+// used.
+ResvSlot &lru_slot(zntt_ctx *c) {
+  ResvSlot *best = &c->resv[0];
+  for (ResvSlot &sl : c->resv) {
+    if (sl.field < 0) return sl;
+    if (sl.last_use < best->last_use) best = &sl;
+  }
+  return *best;
+}
+
+// The table of powers a zig-ntt entry point (ntt, intt) runs over: a cached
+// table that serves it, else, on a context whose reservation covers 2^L, one
+// built into a reservation slot (no allocation), else a new cache entry.
+int call_power_table(zntt_ctx *c, int field, const Elem &root, int L, hipStream_t s, const uint32_t **out) {
+  if (TwiddleEntry *e = find_power_table(c, field, root, L)) {
+    *out = e->table;
+    return ZNTT_OK;
+  }
+  if (L > c->resv_log_n) {
+    TwiddleEntry *e = nullptr;
+    int st = power_table(c, field, root, L, s, &e);
+    if (st == ZNTT_OK) *out = e->table;
+    return st;
+  }
+  for (ResvSlot &sl : c->resv)
+    if (sl.field == field && sl.inv_of == 0 && sl.L >= L && sl.chain[sl.L - L] == root) {
+      sl.last_use = ++c->tick;
+      *out = sl.table;
+      return ZNTT_OK;
+    }
+  ResvSlot &sl = lru_slot(c);
+  sl.field = -1;  // holds nothing until the build succeeds
+  std::vector<Elem> chain = root_chain(field, root, L);
+  hipError_t he = build_powers(field, L, chain_consts(field, chain, L), sl.table, c->resv_consts, s);
+  if (he != hipSuccess) return hip_status(he);
+  sl.field = field;
+  sl.L = L;
+  sl.chain = std::move(chain);
+  sl.inv_of = 0;
+  sl.last_use = ++c->tick;
+  *out = sl.table;
   return ZNTT_OK;
 }
 
@@ -532,6 +621,26 @@ int resolve_twiddles(zntt_ctx *c, int field, size_t log_n, const void *const *st
   TwiddleEntry *reg = find_registered(c, field, log_n, stages);
   if (reg) {
     *tw = reg->table;
+    if (inverse && !reg->inv_table && L <= c->resv_log_n) {
+      // A reserved call: the derived inverse goes to a reservation slot.
+      for (ResvSlot &sl : c->resv)
+        if (sl.field == field && sl.inv_of == reg->reg_id) {
+          sl.last_use = ++c->tick;
+          *tw = sl.table;
+          return ZNTT_OK;
+        }
+      ResvSlot &sl = lru_slot(c);
+      sl.field = -1;
+      hipError_t e = zntt::launch_inverse_table(field, L, sl.table, reg->table, s);
+      if (e != hipSuccess) return hip_status(e);
+      sl.field = field;
+      sl.L = L;
+      sl.chain.clear();
+      sl.inv_of = reg->reg_id;
+      sl.last_use = ++c->tick;
+      *tw = sl.table;
+      return ZNTT_OK;
+    }
     if (inverse) {
       if (!reg->inv_table) {
         reg->inv_bytes = reg->bytes;
@@ -620,6 +729,45 @@ int zntt_ctx_create(zntt_ctx **out, int device, uint64_t hbm_budget_bytes) {
   return ZNTT_OK;
 }
 
+int zntt_ctx_reserve(zntt_ctx *c, size_t max_log_n) {
+  if (!c) return ZNTT_INVALID_ARGUMENT;
+  if (max_log_n > ZNTT_MAX_LOG_N) return ZNTT_UNSUPPORTED;
+  CallGuard g(c);
+  if ((int)max_log_n <= c->resv_log_n) return ZNTT_OK;
+  // The largest element of any field, in the caller's layout and on the device.
+  size_t ub = 0, db = 0;
+  for (int f = 0; f < 16; f++)
+    if (field_ok(f)) {
+      ub = std::max(ub, user_bytes(f));
+      db = std::max(db, dev_bytes(f));
+    }
+  const size_t n = (size_t)1 << max_log_n;
+  // Buffers of the calls at 2^max_log_n: staged data, pass scratch, and a
+  // foreign table's upload (dev table, user staging, derived inverse).
+  int st = ensure(c, &c->staging, &c->staging_bytes, n * ub);
+  if (!st) st = ensure(c, &c->scratch, &c->scratch_bytes, n * db);
+  if (!st) st = ensure(c, &c->aux, &c->aux_bytes, n * (2 * db + ub));
+  const size_t consts = (size_t)(ZNTT_MAX_LOG_N + 2) * 9 * 4;
+  if (!st && !c->resv_consts) {
+    st = dev_alloc(c, consts, (void **)&c->resv_consts);
+    if (!st) c->resv_consts_bytes = consts;
+  }
+  for (ResvSlot &sl : c->resv) {
+    if (st || sl.bytes >= n * db) continue;
+    uint32_t *t = nullptr;
+    st = dev_alloc(c, n * db, (void **)&t);
+    if (st) break;
+    hipStreamSynchronize(c->stream);
+    dev_free(c, sl.table, sl.bytes);
+    sl = ResvSlot();
+    sl.table = t;
+    sl.bytes = n * db;
+  }
+  if (st) return st;
+  c->resv_log_n = (int)max_log_n;
+  return ZNTT_OK;
+}
+
 int zntt_ctx_trim(zntt_ctx *c) {
   if (!c) return ZNTT_INVALID_ARGUMENT;
   CallGuard g(c);
@@ -693,11 +841,11 @@ int zntt_ntt(zntt_ctx *c, int field, void *data, size_t len, size_t log_n, const
   if (!root) return ZNTT_INVALID_ARGUMENT;
   CallGuard g(c, c->stream);
   if (log_n == 0) return ZNTT_OK;
-  TwiddleEntry *t = nullptr;
-  st = power_table(c, field, load_elem(field, root), (int)log_n, c->stream, &t);
+  const uint32_t *tw = nullptr;
+  st = call_power_table(c, field, load_elem(field, root), (int)log_n, c->stream, &tw);
   if (st) return st;
   return with_staging(c, field, data, len, [&](void *d) {
-    return run_device(c, field, (int)log_n, d, t->table, nullptr, nullptr, c->stream);
+    return run_device(c, field, (int)log_n, d, tw, nullptr, nullptr, c->stream);
   });
 }
 
@@ -708,12 +856,12 @@ int zntt_intt(zntt_ctx *c, int field, void *data, size_t len, size_t log_n, cons
   CallGuard g(c, c->stream);
   if (log_n == 0) return ZNTT_OK;  // n^-1 = 1
   const Elem root_inv = f_inv(field, load_elem(field, root));  // root.zig:100
-  TwiddleEntry *t = nullptr;
-  st = power_table(c, field, root_inv, (int)log_n, c->stream, &t);
+  const uint32_t *tw = nullptr;
+  st = call_power_table(c, field, root_inv, (int)log_n, c->stream, &tw);
   if (st) return st;
   const std::array<uint32_t, 9> scale = f_to_dev(field, f_inv(field, f_from_u64(field, len)));  // root.zig:103
   return with_staging(c, field, data, len, [&](void *d) {
-    return run_device(c, field, (int)log_n, d, t->table, nullptr, &scale, c->stream);
+    return run_device(c, field, (int)log_n, d, tw, nullptr, &scale, c->stream);
   });
 }
 
@@ -769,6 +917,7 @@ int zntt_precompute_twiddles(zntt_ctx *c, int field, size_t log_n, const void *r
   for (size_t s = 0; s < log_n; s++) memcpy(stages[s], host.data() + (((size_t)1 << s) - 1) * ub, ((size_t)1 << s) * ub);
   e.key.assign(stages, stages + log_n);
   e.fingerprint = table_fingerprint(field, log_n, (const void *const *)stages);
+  e.reg_id = c->next_reg_id++;
   e.last_use = ++c->tick;
   c->tables.push_back(std::move(e));
   return ZNTT_OK;

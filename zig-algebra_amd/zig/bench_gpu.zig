@@ -25,7 +25,7 @@ var gpu_sink: usize = 0;
 fn nttGpuSetup() void {
     if (gpu_buf.len != 0) return;
     const F = FrScalar;
-    gpu_ctx = gpu.GpuNtt.init(0) catch |e| std.debug.panic("GpuNtt.init failed: {s}", .{@errorName(e)});
+    gpu_ctx = gpu.GpuNtt.init(0, 20) catch |e| std.debug.panic("GpuNtt.init failed: {s}", .{@errorName(e)});
     gpu_buf = std.heap.page_allocator.alloc(F, 1 << 20) catch @panic("oom");
     for (gpu_buf, 0..) |*x, i| x.* = F.fromInt(@as(u64, i % 1000));
     const root = F.rootOfUnity(gpu_buf.len) catch |e| std.debug.panic("rootOfUnity(2^20) failed: {s}", .{@errorName(e)});
@@ -55,7 +55,7 @@ pub fn checkAgainstCpu() !void {
     }
     const root = try F.rootOfUnity(a.len);
     try ntt_mod.ntt(F, a, 12, root);
-    var g = try gpu.GpuNtt.init(0);
+    var g = try gpu.GpuNtt.init(0, 12);
     defer g.deinit();
     try g.ntt(F, b, 12, root);
     if (!std.mem.eql(u8, std.mem.sliceAsBytes(a), std.mem.sliceAsBytes(b))) return error.Mismatch;

@@ -19,6 +19,7 @@ const std = @import("std");
 pub const Ctx = opaque {};
 extern fn zntt_ctx_create(out: *?*Ctx, device: c_int, hbm_budget: u64) c_int;
 extern fn zntt_ctx_destroy(ctx: *Ctx) void;
+extern fn zntt_ctx_reserve(ctx: *Ctx, max_log_n: usize) c_int;
 extern fn zntt_bit_reverse(ctx: *Ctx, field: c_int, data: *anyopaque, len: usize) c_int;
 extern fn zntt_ntt(ctx: *Ctx, field: c_int, data: *anyopaque, len: usize, log_n: usize, root: *const anyopaque) c_int;
 extern fn zntt_intt(ctx: *Ctx, field: c_int, data: *anyopaque, len: usize, log_n: usize, root: *const anyopaque) c_int;
@@ -53,18 +54,40 @@ fn fieldId(comptime F: type) c_int {
 pub const GpuNtt = struct {
     ctx: *Ctx,
 
-    pub fn init(device: c_int) !GpuNtt {
+    // max_log_n: the largest transform the caller will run.  Everything the
+    // seven functions need up to 2^max_log_n is allocated here
+    // (zntt_ctx_reserve), so a full device surfaces as error.OutOfMemory from
+    // init and never from a transform.
+    pub fn init(device: c_int, max_log_n: usize) !GpuNtt {
         var c: ?*Ctx = null;
         if (zntt_ctx_create(&c, device, 0) != 0) return error.NoDevice;
+        switch (zntt_ctx_reserve(c.?, max_log_n)) {
+            0 => {},
+            5 => {
+                zntt_ctx_destroy(c.?);
+                return error.OutOfMemory;
+            },
+            9 => {
+                zntt_ctx_destroy(c.?);
+                return error.Unsupported;
+            },
+            else => {
+                zntt_ctx_destroy(c.?);
+                return error.DeviceError;
+            },
+        }
         return .{ .ctx = c.? };
     }
     pub fn deinit(self: GpuNtt) void {
         zntt_ctx_destroy(self.ctx);
     }
 
-    // Device-only statuses (DeviceError = 6, Unsupported = 9) are not in
-    // zig-ntt's error sets; the shim panics on them so every signature below
-    // keeps the reference's exact error set.
+    // Device-only statuses (OutOfMemory = 5 from a transform, DeviceError =
+    // 6, Unsupported = 9) are not in zig-ntt's error sets.  After init's
+    // reservation a transform at log_n <= max_log_n allocates nothing, so
+    // they come only from a HIP failure or a size past the reservation; the
+    // shim panics on them so every signature below keeps the reference's
+    // exact error set.
     fn device(st: c_int) noreturn {
         std.debug.panic("zig-ntt GPU backend: status {d}", .{st});
     }

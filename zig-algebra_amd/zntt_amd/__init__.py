@@ -96,7 +96,8 @@ _ERRORS = {c.status: c for c in (LogTooLarge, LengthMismatch, InvalidLength, Inv
 
 # Every symbol include/zntt_gpu.h declares.
 EXPORTED = [
-    "zntt_ctx_create", "zntt_ctx_destroy", "zntt_ctx_bytes_in_use", "zntt_ctx_trim", "zntt_status_string",
+    "zntt_ctx_create", "zntt_ctx_destroy", "zntt_ctx_bytes_in_use", "zntt_ctx_trim", "zntt_ctx_reserve",
+    "zntt_status_string",
     "zntt_element_size", "zntt_bit_reverse", "zntt_ntt", "zntt_intt", "zntt_precompute_twiddles",
     "zntt_free_twiddles", "zntt_ntt_with_twiddles", "zntt_intt_with_twiddles", "zntt_coset_ntt", "zntt_lde",
     "zntt_primitive_root", "zntt_execute_device", "zntt_ctx_synchronize", "zntt_debug_field_op",
@@ -191,6 +192,7 @@ def _load():
     L.zntt_ctx_bytes_in_use.argtypes = [vp]
     L.zntt_ctx_bytes_in_use.restype = u64
     L.zntt_ctx_trim.argtypes = [vp]
+    L.zntt_ctx_reserve.argtypes = [vp, ctypes.c_size_t]
     L.zntt_status_string.argtypes = [i32]
     L.zntt_status_string.restype = ctypes.c_char_p
     L.zntt_element_size.argtypes = [i32]
@@ -417,6 +419,12 @@ class Context:
 
     def trim(self) -> None:
         check(lib.zntt_ctx_trim(self._h))
+
+    def reserve(self, max_log_n: int) -> None:
+        """Allocates up front what the seven zig-ntt functions use up to
+        2^max_log_n, so that those calls allocate no device memory after it
+        (zntt_ctx_reserve)."""
+        check(lib.zntt_ctx_reserve(self._h, max_log_n))
 
     def synchronize(self) -> None:
         check(lib.zntt_ctx_synchronize(self._h))
