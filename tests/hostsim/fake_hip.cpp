@@ -307,7 +307,7 @@ hipError_t launch_batch_level(int field, const BatchXform *xf, const BatchEntry 
   });
 }
 
-hipError_t launch_batch_level_r8(int field, int, bool, int, const BatchXform *xf, const BatchEntry *ent,
+hipError_t launch_batch_level_r8(int field, int, bool, bool, int, const BatchXform *xf, const BatchEntry *ent,
                                  uint32_t nent, uint32_t nitems, uint32_t uniform, uint32_t *counter, int grid,
                                  hipStream_t s) {
   return launch_batch_level(field, xf, ent, nent, nitems, uniform, counter, grid, 0, s);

@@ -106,6 +106,23 @@ def test_mixed_requests_equal_alone_and_oracle(Z, ctx, field):
     srv.close()
 
 
+@pytest.mark.parametrize("field", ALL, ids=IDS)
+def test_padding_deeper_than_the_skipped_stages(Z, ctx, field):
+    """The level kernel skips the products of zero padding in at most the
+    first three stages of a request's first pass (blowup up to 8); a
+    blowup of 2, 16 or 32 pads 1, 4 or 5 stages.  Multi-pass requests with
+    those blowups, served together, equal the oracle's LDE bit for bit."""
+    shifts = [O.from_int(field, 5), O.from_int(field, 7)]
+    reqs = [(10, 4, 0, 601), (11, 5, 1, 602), (12, 1, 0, 603), (9, 4, 1, 604), (13, 3, 0, 605)]
+    srv = Z.Server(ctx, field, hbm_budget=1 << 30, max_log_n=16)
+    ts = [srv.submit(k, b, shifts[si], seed=4, stream_id=sid, keep=True) for k, b, si, sid in reqs]
+    srv.drain()
+    for (k, b, si, sid), t in zip(reqs, ts):
+        assert np.array_equal(srv.read(t), expected(field, 4, sid, k, b, shifts[si])), (k, b)
+        srv.release(t)
+    srv.close()
+
+
 def test_budget_forces_queueing_results_exact(Z, ctx):
     """A budget that holds two 2^14-evaluation requests at a time."""
     F = O.FIELD_FR

@@ -59,19 +59,22 @@ constexpr int kWholeMinLog = 9;
 
 // The register-round level kernel (level_r8_kernel): one persistent launch
 // over tiles of 2^tile_log elements whose passes all have K stages and first
-// stages of one parity.  level_r8_ok: whether a pass of K stages with that
-// tile can take it (K in 5..kLevelMaxK[field], 2^tile_log / N threads in
-// 64..512, N = 4 values a thread for BN254 Fr, 8 for the others).
-hipError_t launch_batch_level_r8(int field, int K, bool odd, int tile_log, const BatchXform *xf,
+// stages of one parity.  zero_pad: some first pass of the launch reads a
+// source shorter than its transform (src_len < 2^L, zero padding); the
+// launch then uses level_r8_pad_kernel, which skips the products of the
+// padding.  level_r8_ok: whether a pass of K stages with that tile can take
+// it (K in 5..kLevelMaxK[field], 2^tile_log / N threads in 64..512, N = 4
+// values a thread for BN254 Fr, 8 for the others).
+hipError_t launch_batch_level_r8(int field, int K, bool odd, bool zero_pad, int tile_log, const BatchXform *xf,
                                  const BatchEntry *ent, uint32_t nent, uint32_t nitems, uint32_t uniform_tiles,
                                  uint32_t *counter, int grid, hipStream_t s);
-hipError_t launch_level_r8_fr(int K, bool odd, int tile_log, const BatchXform *xf, const BatchEntry *ent,
-                              uint32_t nent, uint32_t nitems, uint32_t uniform_tiles, uint32_t *counter, int grid,
-                              hipStream_t s);
-hipError_t launch_level_r8_gl(int K, bool odd, int tile_log, const BatchXform *xf, const BatchEntry *ent,
-                              uint32_t nent, uint32_t nitems, uint32_t uniform_tiles, uint32_t *counter, int grid,
-                              hipStream_t s);
-hipError_t launch_level_r8_f31(int field, int K, bool odd, int tile_log, const BatchXform *xf,
+hipError_t launch_level_r8_fr(int K, bool odd, bool zero_pad, int tile_log, const BatchXform *xf,
+                              const BatchEntry *ent, uint32_t nent, uint32_t nitems, uint32_t uniform_tiles,
+                              uint32_t *counter, int grid, hipStream_t s);
+hipError_t launch_level_r8_gl(int K, bool odd, bool zero_pad, int tile_log, const BatchXform *xf,
+                              const BatchEntry *ent, uint32_t nent, uint32_t nitems, uint32_t uniform_tiles,
+                              uint32_t *counter, int grid, hipStream_t s);
+hipError_t launch_level_r8_f31(int field, int K, bool odd, bool zero_pad, int tile_log, const BatchXform *xf,
                                const BatchEntry *ent, uint32_t nent, uint32_t nitems, uint32_t uniform_tiles,
                                uint32_t *counter, int grid, hipStream_t s);
 inline bool level_r8_ok(int field, int K, int tile_log) {

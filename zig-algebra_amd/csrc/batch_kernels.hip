@@ -236,14 +236,17 @@ hipError_t launch_batch_level(int field, const BatchXform *xf, const BatchEntry 
   return hipGetLastError();
 }
 
-hipError_t launch_batch_level_r8(int field, int K, bool odd, int tile_log, const BatchXform *xf,
+hipError_t launch_batch_level_r8(int field, int K, bool odd, bool zero_pad, int tile_log, const BatchXform *xf,
                                  const BatchEntry *ent, uint32_t nent, uint32_t nitems, uint32_t uniform_tiles,
                                  uint32_t *counter, int grid, hipStream_t s) {
   if (!level_r8_ok(field, K, tile_log)) return hipErrorInvalidValue;
   if (uniform_tiles && (uint64_t)uniform_tiles * nent != nitems) return hipErrorInvalidValue;
-  if (field == kFr) return launch_level_r8_fr(K, odd, tile_log, xf, ent, nent, nitems, uniform_tiles, counter, grid, s);
-  if (field == kGl) return launch_level_r8_gl(K, odd, tile_log, xf, ent, nent, nitems, uniform_tiles, counter, grid, s);
-  return launch_level_r8_f31(field, K, odd, tile_log, xf, ent, nent, nitems, uniform_tiles, counter, grid, s);
+  if (field == kFr)
+    return launch_level_r8_fr(K, odd, zero_pad, tile_log, xf, ent, nent, nitems, uniform_tiles, counter, grid, s);
+  if (field == kGl)
+    return launch_level_r8_gl(K, odd, zero_pad, tile_log, xf, ent, nent, nitems, uniform_tiles, counter, grid, s);
+  return launch_level_r8_f31(field, K, odd, zero_pad, tile_log, xf, ent, nent, nitems, uniform_tiles, counter, grid,
+                             s);
 }
 
 hipError_t configure_batch_lds() {

@@ -75,6 +75,10 @@ struct M31T {
     A::butterfly_one(u, v);
   }
   static constexpr bool one_ok(int) { return true; }
+  template <bool NORM = true>
+  __device__ static void bfly_zero(E &u, E &v) {
+    v = u;
+  }
   __device__ static E canon(E x) { return A::canon(x); }
   __device__ static E neg(const E &x) { return x == 0 ? 0 : A::P - x; }
 };

@@ -112,6 +112,14 @@ struct FrT {
       fr_butterfly_one<NORM>(u, v);
   }
   static constexpr bool one_ok(int T) { return T <= 1; }
+  // Butterfly whose v is zero (the first stages over a zero-padded input):
+  // (u + w 0, u - w 0) = (u, u), with no product.  The copy keeps u's bound,
+  // below what a full butterfly would give.
+  template <bool NORM = true>
+  __device__ static void bfly_zero(E &u, E &v) {
+    if (NORM) fr_norm(u);
+    v = u;
+  }
   // Canonical dev-form constant (for tables).
   __device__ static E canon(E x) {
     fr_canon(x);
@@ -170,6 +178,10 @@ struct GlT {
     gl_butterfly_one(u, v);
   }
   static constexpr bool one_ok(int) { return true; }
+  template <bool NORM = true>
+  __device__ static void bfly_zero(E &u, E &v) {
+    v = u;
+  }
   __device__ static E canon(E x) { return gl_canon(x); }
   __device__ static E neg(const E &x) { return gl_neg(x); }
 };
@@ -214,6 +226,10 @@ struct F31T {
     A::butterfly_one(u, v);
   }
   static constexpr bool one_ok(int) { return true; }
+  template <bool NORM = true>
+  __device__ static void bfly_zero(E &u, E &v) {
+    v = u;
+  }
   __device__ static E canon(E x) { return A::canon(x); }
   __device__ static E neg(const E &x) { return x == 0 ? 0 : P - x; }
 };

@@ -29,11 +29,13 @@ hipError_t launch_whole_f31(int field, int K, const BatchXform *xf, const BatchE
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_level_r8_f31(int field, int K, bool odd, int tile_log, const BatchXform *xf,
+hipError_t launch_level_r8_f31(int field, int K, bool odd, bool zero_pad, int tile_log, const BatchXform *xf,
                                const BatchEntry *ent, uint32_t nent, uint32_t nitems, uint32_t uniform,
                                uint32_t *counter, int grid, hipStream_t s) {
-  if (field == kBb) return launch_level_t<BbT, 10>(K, odd, tile_log, xf, ent, nent, nitems, uniform, counter, grid, s);
-  if (field == kKb) return launch_level_t<KbT, 10>(K, odd, tile_log, xf, ent, nent, nitems, uniform, counter, grid, s);
+  if (field == kBb)
+    return launch_level_t<BbT, 10>(K, odd, zero_pad, tile_log, xf, ent, nent, nitems, uniform, counter, grid, s);
+  if (field == kKb)
+    return launch_level_t<KbT, 10>(K, odd, zero_pad, tile_log, xf, ent, nent, nitems, uniform, counter, grid, s);
   return hipErrorInvalidValue;
 }
 

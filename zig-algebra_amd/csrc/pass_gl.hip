@@ -19,10 +19,10 @@ hipError_t launch_whole_gl(int K, const BatchXform *xf, const BatchEntry *ent, u
   return launch_whole_t<GlT>(K, xf, ent, n, counter, tw_all, s);
 }
 
-hipError_t launch_level_r8_gl(int K, bool odd, int tile_log, const BatchXform *xf, const BatchEntry *ent,
-                              uint32_t nent, uint32_t nitems, uint32_t uniform, uint32_t *counter, int grid,
-                              hipStream_t s) {
-  return launch_level_t<GlT, 10>(K, odd, tile_log, xf, ent, nent, nitems, uniform, counter, grid, s);
+hipError_t launch_level_r8_gl(int K, bool odd, bool zero_pad, int tile_log, const BatchXform *xf,
+                              const BatchEntry *ent, uint32_t nent, uint32_t nitems, uint32_t uniform,
+                              uint32_t *counter, int grid, hipStream_t s) {
+  return launch_level_t<GlT, 10>(K, odd, zero_pad, tile_log, xf, ent, nent, nitems, uniform, counter, grid, s);
 }
 
 }  // namespace zntt

@@ -224,6 +224,18 @@ struct StageMap {
   static constexpr int j(int jj) { return kk(jj) & ((1 << T) - 1); }
 };
 
+// Zero padding of a first pass: z when the source holds 2^(L - z) values
+// (src_len a power of two below 2^L) and the other inputs are zero; else 0.
+// Padded kernels test for it in the first kPadStages stages only (blowups up
+// to 8; a deeper padding runs its further stages as ordinary butterflies):
+// the test in every round costs the register-round kernels registers.
+constexpr int kPadStages = 3;
+__device__ __forceinline__ int pad_log(const PassArgs &a) {
+  const uint64_t len = a.src_len;
+  if (!a.first || len == 0 || (len & (len - 1)) != 0 || len >= ((uint64_t)1 << a.L)) return 0;
+  return a.L - (63 - __builtin_clzll(len));
+}
+
 // TWL: the pass's twiddle table is resident in LDS after the tile's planes
 // (a whole transform: 2^K entries, plane stride 2^K), not read from HBM.
 // WHOLE: the round runs in the whole-transform batch kernel.
@@ -234,13 +246,20 @@ struct StageMap {
 // words.  Positions and twiddle indices use the column without the lane bits
 // (c.c0, c.lo0 count such columns); LDS slots use the full column.
 // PAD: the first pass reads zero at and past a.src_len, from the source and
-// from the coset table (a first pass that zero-pads its input).
+// from the coset table (a first pass that zero-pads its input).  ZSKIP (with
+// PAD): when the source is 2^(L - z) values (pad_log), the inputs at
+// bit-reversed positions whose low z bits are not all zero are the padding,
+// so every butterfly of stages 0 .. z-1 has v = 0 and becomes (u, u), and
+// coset products of padding are skipped where a whole wave holds only
+// padding.
 template <class F, int R, int l0, bool first_round, bool last_round, bool ODD, bool TWL = false, int N = 8,
-          bool WHOLE = false, int LB = 0, bool PAD = false>
+          bool WHOLE = false, int LB = 0, bool PAD = false, bool ZSKIP = false>
 __device__ __forceinline__ void pass_round(typename F::E (&x)[N], const PassArgs &a, uint32_t *lds,
                                            const TileCtx &c) {
   using E = typename F::E;
   constexpr int SET = 1 << R, NSETS = N >> R;
+  int zlog = 0;  // zero-padding stages of this round's transform (first pass only)
+  if constexpr (PAD && ZSKIP) zlog = pad_log(a);
   int g[NSETS], base[NSETS], gr[NSETS], ln[NSETS];
   sfor<NSETS>([&](auto i) {
     const int slot = threadIdx.x + i * blockDim.x;
@@ -324,6 +343,16 @@ __device__ __forceinline__ void pass_round(typename F::E (&x)[N], const PassArgs
       };
       sfor<N / 2>([&](auto h) {
         constexpr int e = 2 * decltype(h)::value;
+        if constexpr (PAD && ZSKIP) {
+          // Skip the pair's products when both elements are padding in
+          // every lane of the wave (their values are zero).
+          auto live = [&](int f) {
+            const int i = f / SET, k = f % SET;
+            const int mid = base[i] | (k << l0);
+            return ((size_t)brev_bits(mid, c.K) << c.hi_bits) + c.c0 + gr[i] < a.src_len;
+          };
+          if (zlog > 0 && !__any(live(e) || live(e + 1))) return;
+        }
         F::mul2(x[e], cos(e), x[e + 1], cos(e + 1));
       });
     }
@@ -334,6 +363,15 @@ __device__ __forceinline__ void pass_round(typename F::E (&x)[N], const PassArgs
     constexpr int T = decltype(t)::value;
     constexpr bool norm = (((ODD ? 1 : 0) + l0 + T) & 1) != 0;
     using M = StageMap<R, T>;
+    if constexpr (PAD && ZSKIP && l0 + T < kPadStages) {
+      if (a.first && l0 + T < zlog) {  // v = 0 in every butterfly of this stage
+        sfor<N / 2>([&](auto j) {
+          constexpr int jj = decltype(j)::value;
+          F::template bfly_zero<norm>(x[M::u(jj)], x[M::u(jj) + (1 << T)]);
+        });
+        return;
+      }
+    }
     sfor<N / 4>([&](auto h) {
       constexpr int j0 = 2 * decltype(h)::value, j1 = j0 + 1;
       auto both = [&] {
@@ -399,7 +437,7 @@ __device__ __forceinline__ void pass_round(typename F::E (&x)[N], const PassArgs
 // K - LN*(ceil(K/LN) - 1) stages, the rest LN each; unrolled at compile time
 // so `x` stays in registers.
 template <class F, int K, int RI, bool ODD, bool TWL = false, int N = 8, bool WHOLE = false, int LB = 0,
-          bool PAD = false>
+          bool PAD = false, bool ZSKIP = false>
 __device__ __forceinline__ void pass_rounds(typename F::E (&x)[N], const PassArgs &a, uint32_t *lds,
                                             const TileCtx &c) {
   constexpr int LN = N == 8 ? 3 : 2;
@@ -407,10 +445,10 @@ __device__ __forceinline__ void pass_rounds(typename F::E (&x)[N], const PassArg
   constexpr int R0 = K - LN * (NR - 1);
   constexpr int R = RI == 0 ? R0 : LN;
   constexpr int L0 = RI == 0 ? 0 : R0 + LN * (RI - 1);
-  pass_round<F, R, L0, RI == 0, RI == NR - 1, ODD, TWL, N, WHOLE, LB, PAD>(x, a, lds, c);
+  pass_round<F, R, L0, RI == 0, RI == NR - 1, ODD, TWL, N, WHOLE, LB, PAD, ZSKIP>(x, a, lds, c);
   if constexpr (RI + 1 < NR) {
     __syncthreads();
-    pass_rounds<F, K, RI + 1, ODD, TWL, N, WHOLE, LB, PAD>(x, a, lds, c);
+    pass_rounds<F, K, RI + 1, ODD, TWL, N, WHOLE, LB, PAD, ZSKIP>(x, a, lds, c);
   }
 }
 
@@ -491,11 +529,12 @@ __device__ __forceinline__ uint32_t find_entry(const BatchEntry *ent, uint32_t n
 // Items are claimed from next[0] as in the other persistent launches
 // (next[0..2) zero at launch and again when it ends).  A first pass reads
 // zero at and past its transform's src_len, from the source and from the
-// coset table.
-template <class F, int K, bool ODD>
-__global__ __launch_bounds__(512) void level_r8_kernel(const BatchXform *__restrict__ xf,
-                                                       const BatchEntry *__restrict__ ent, uint32_t nent,
-                                                       uint32_t nitems, uint32_t uniform_tiles, uint32_t *next) {
+// coset table; with ZSKIP it also skips the products of that padding
+// (pass_round).
+template <class F, int K, bool ODD, bool ZSKIP>
+__device__ __forceinline__ void level_r8_body(const BatchXform *__restrict__ xf, const BatchEntry *__restrict__ ent,
+                                              uint32_t nent, uint32_t nitems, uint32_t uniform_tiles,
+                                              uint32_t *next) {
   using E = typename F::E;
   constexpr int N = round_elems<F>();
   extern __shared__ uint32_t lds[];
@@ -543,13 +582,31 @@ __global__ __launch_bounds__(512) void level_r8_kernel(const BatchXform *__restr
       c.lo0 = ((size_t)tile_id & (((size_t)1 << lbb) - 1)) << c.lg;
     }
     E x[N];
-    pass_rounds<F, K, 0, ODD, false, N, false, 0, true>(x, a, lds, c);
+    pass_rounds<F, K, 0, ODD, false, N, false, 0, true, ZSKIP>(x, a, lds, c);
     __syncthreads();  // LDS and s_pos are reused by the next item
     if (threadIdx.x == 0) s_item = claimed;
     __syncthreads();
     it = __builtin_amdgcn_readfirstlane(s_item);
   }
   if (threadIdx.x == 0) retire_workgroup(next);
+}
+
+template <class F, int K, bool ODD>
+__global__ __launch_bounds__(512) void level_r8_kernel(const BatchXform *__restrict__ xf,
+                                                       const BatchEntry *__restrict__ ent, uint32_t nent,
+                                                       uint32_t nitems, uint32_t uniform_tiles, uint32_t *next) {
+  level_r8_body<F, K, ODD, false>(xf, ent, nent, nitems, uniform_tiles, next);
+}
+
+// The level kernel for launches whose first passes may zero-pad (served LDE
+// requests): padding products skipped.  Four waves a SIMD (at most 128
+// VGPRs, as the LDS tile allows for every field); the skip's branches would
+// otherwise take BN254 Fr's odd-K kernels to 129-130 VGPRs and three waves.
+template <class F, int K, bool ODD>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void level_r8_pad_kernel(
+    const BatchXform *__restrict__ xf, const BatchEntry *__restrict__ ent, uint32_t nent, uint32_t nitems,
+    uint32_t uniform_tiles, uint32_t *next) {
+  level_r8_body<F, K, ODD, true>(xf, ent, nent, nitems, uniform_tiles, next);
 }
 
 // Waves a SIMD that the whole-transform kernel asks the register allocator
@@ -677,13 +734,14 @@ static hipError_t launch_whole_t(int K, const BatchXform *xf, const BatchEntry *
   }
 }
 
-// One level_r8_kernel launch over `nitems` tiles of 2^tile_log elements
-// (K + each pass's log_g = tile_log), at most `grid` workgroups and at most
-// as many as are resident at once.
-template <class F, int K, bool ODD>
+// One level_r8_kernel launch (level_r8_pad_kernel with ZP) over `nitems`
+// tiles of 2^tile_log elements (K + each pass's log_g = tile_log), at most
+// `grid` workgroups and at most as many as are resident at once.
+template <class F, int K, bool ODD, bool ZP>
 static hipError_t launch_level_k(int tile_log, const BatchXform *xf, const BatchEntry *ent, uint32_t nent,
                                  uint32_t nitems, uint32_t uniform, uint32_t *counter, int grid, hipStream_t s) {
   constexpr int N = round_elems<F>();
+  auto *const kern = ZP ? &level_r8_pad_kernel<F, K, ODD> : &level_r8_kernel<F, K, ODD>;
   const unsigned threads = (1u << tile_log) / N;
   const size_t lds = (size_t)F::W * 4 << tile_log;
   // Resident workgroups per CU for each tile size, computed once.
@@ -692,8 +750,7 @@ static hipError_t launch_level_k(int tile_log, const BatchXform *xf, const Batch
   static int cus = 0;
   static hipError_t setup = hipSuccess;
   std::call_once(once, [&] {
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&level_r8_kernel<F, K, ODD>),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                                        F::W * 4 << (std::is_same<F, FrT>::value ? 11 : 12));
     int dev = 0;
     if (e == hipSuccess) e = hipGetDevice(&dev);
@@ -701,8 +758,7 @@ static hipError_t launch_level_k(int tile_log, const BatchXform *xf, const Batch
     for (int tl = 8; tl <= 12 && e == hipSuccess; tl++) {
       const unsigned t = (1u << tl) / N;
       if (t < 64 || t > 512) continue;
-      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu[tl], level_r8_kernel<F, K, ODD>, (int)t,
-                                                       (size_t)F::W * 4 << tl);
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu[tl], kern, (int)t, (size_t)F::W * 4 << tl);
     }
     setup = e;
   });
@@ -711,27 +767,26 @@ static hipError_t launch_level_k(int tile_log, const BatchXform *xf, const Batch
     return hipErrorInvalidValue;
   const int g = std::min<int>(std::min<int>(grid, per_cu[tile_log] * cus), (int)nitems);
   if (g < 1) return hipSuccess;
-  hipLaunchKernelGGL((level_r8_kernel<F, K, ODD>), dim3(g), dim3(threads), lds, s, xf, ent, nent, nitems, uniform,
-                     counter);
+  hipLaunchKernelGGL(kern, dim3(g), dim3(threads), lds, s, xf, ent, nent, nitems, uniform, counter);
   return hipGetLastError();
 }
 
-// Pass lengths 5..KMAX.
+// Pass lengths 5..KMAX; zero_pad selects the padding-skipping kernel.
 template <class F, int KMAX>
-static hipError_t launch_level_t(int K, bool odd, int tile_log, const BatchXform *xf, const BatchEntry *ent,
-                                 uint32_t nent, uint32_t nitems, uint32_t uniform, uint32_t *counter, int grid,
-                                 hipStream_t s) {
+static hipError_t launch_level_t(int K, bool odd, bool zero_pad, int tile_log, const BatchXform *xf,
+                                 const BatchEntry *ent, uint32_t nent, uint32_t nitems, uint32_t uniform,
+                                 uint32_t *counter, int grid, hipStream_t s) {
   if (nitems == 0) return hipSuccess;
-  switch (K * 2 + (odd ? 1 : 0)) {
-#define ZNTT_LV(k)                                                                                           \
-  case 2 * k:                                                                                              \
-    if constexpr (k <= KMAX) return launch_level_k<F, k, false>(tile_log, xf, ent, nent, nitems, uniform, counter, grid, s); \
-    break;                                                                                                 \
-  case 2 * k + 1:                                                                                          \
-    if constexpr (k <= KMAX) return launch_level_k<F, k, true>(tile_log, xf, ent, nent, nitems, uniform, counter, grid, s); \
+  switch (K * 4 + (odd ? 2 : 0) + (zero_pad ? 1 : 0)) {
+#define ZNTT_LV1(k, o, z)                                                                                  \
+  case 4 * k + 2 * o + z:                                                                                \
+    if constexpr (k <= KMAX)                                                                             \
+      return launch_level_k<F, k, o != 0, z != 0>(tile_log, xf, ent, nent, nitems, uniform, counter, grid, s); \
     break;
+#define ZNTT_LV(k) ZNTT_LV1(k, 0, 0) ZNTT_LV1(k, 0, 1) ZNTT_LV1(k, 1, 0) ZNTT_LV1(k, 1, 1)
     ZNTT_LV(5) ZNTT_LV(6) ZNTT_LV(7) ZNTT_LV(8) ZNTT_LV(9) ZNTT_LV(10)
 #undef ZNTT_LV
+#undef ZNTT_LV1
     default: break;
   }
   return hipErrorInvalidValue;

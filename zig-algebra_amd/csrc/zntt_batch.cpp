@@ -229,8 +229,8 @@ hipError_t launch_one(zntt_batch *b, size_t i, hipStream_t s) {
       return zntt::launch_batch_level(b->field, b->d_xf, ent, l.nent, l.nitems, l.uniform, ctr, b->grid,
                                       b->shape.tile_log, s);
     case zntt_batch::kLevelR8:
-      return zntt::launch_batch_level_r8(b->field, l.K, l.odd != 0, b->shape.tile_log, b->d_xf, ent, l.nent,
-                                         l.nitems, l.uniform, ctr, b->grid, s);
+      return zntt::launch_batch_level_r8(b->field, l.K, l.odd != 0, false, b->shape.tile_log, b->d_xf, ent,
+                                         l.nent, l.nitems, l.uniform, ctr, b->grid, s);
     default:
       if (b->field == ZNTT_FIELD_BN254_FR) return zntt::launch_whole_fr(l.whole_log, b->d_xf, ent, l.nent, ctr, l.tw_all, s);
       if (b->field == ZNTT_FIELD_GOLDILOCKS) return zntt::launch_whole_gl(l.whole_log, b->d_xf, ent, l.nent, ctr, l.tw_all, s);

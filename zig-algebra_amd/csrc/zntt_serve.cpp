@@ -551,6 +551,15 @@ struct HipDevice : Device {
   };
   static constexpr int kMaxGroups = 32;  // counter pairs in a step's 256-byte header
 
+  // Whether a launch holds the first pass of a request whose coefficients
+  // are fewer than its evaluations (zero padding): the level kernel then
+  // skips the padding's products.
+  static bool pads_first_pass(const Group &g) {
+    for (const StepItem *it : g.items)
+      if (it->pass == 0 && (int)it->r->req.log_n < it->r->L) return true;
+    return false;
+  }
+
   static int s0_of(const Request *r, int p) {
     int s0 = 1;
     for (int q = 0; q < p; q++) s0 += r->plan.K[q];
@@ -656,8 +665,8 @@ struct HipDevice : Device {
         e = zntt::launch_batch_level(field, dx, de + l.ent_off, (uint32_t)l.nent, l.nitems, l.uniform, ctr, grid,
                                      tile_log, st);
       else if (g.kind == kLevelR8)
-        e = zntt::launch_batch_level_r8(field, g.a, g.b != 0, tile_log, dx, de + l.ent_off, (uint32_t)l.nent,
-                                        l.nitems, l.uniform, ctr, grid, st);
+        e = zntt::launch_batch_level_r8(field, g.a, g.b != 0, pads_first_pass(g), tile_log, dx, de + l.ent_off,
+                                        (uint32_t)l.nent, l.nitems, l.uniform, ctr, grid, st);
       else if (field == ZNTT_FIELD_BN254_FR)
         e = zntt::launch_whole_fr(g.a, dx, de + l.ent_off, (uint32_t)l.nent, ctr, nullptr, st);
       else if (field == ZNTT_FIELD_GOLDILOCKS)
