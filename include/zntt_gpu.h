@@ -307,7 +307,10 @@ void zntt_batch_destroy(zntt_batch *b);
 /* launches (requests that fit one tile by size; passes of larger requests   */
 /* by pass length and parity, so large requests share launches; the rest     */
 /* together), run as parallel branches of the server's stream.  A request    */
-/* admitted while others are on the device joins the next step.              */
+/* admitted while others are on the device joins the next step.  A first     */
+/* pass reads only the request's 2^log_n coefficients and skips the          */
+/* products of the zero padding (stages 1..log_blowup of the bit-reversed    */
+/* input copy their u, up to blowup 8).                                      */
 /* A request's result equals zntt_lde on the same coefficients.              */
 /* ------------------------------------------------------------------------ */
 typedef struct zntt_server zntt_server;
