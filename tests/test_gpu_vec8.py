@@ -33,7 +33,7 @@ def rand(n, seed):
     return np.random.default_rng(seed).integers(0, P, size=n, dtype=np.uint64)
 
 
-@pytest.mark.parametrize("log_n", [0, 1, 2, 5, 6, 7, 8, 9, 10, 11, 12, 13, 16, 20])
+@pytest.mark.parametrize("log_n", [0, 1, 2, 5, 6, 7, 8, 9, 10, 11, 12, 13, 16, 17, 20])
 @pytest.mark.parametrize("inverse", [False, True], ids=["fwd", "inv"])
 def test_host_matches_oracle(Z, ctx, log_n, inverse):
     n = 1 << log_n
@@ -46,7 +46,7 @@ def test_host_matches_oracle(Z, ctx, log_n, inverse):
         assert np.array_equal(got, want), (log_n, root)
 
 
-@pytest.mark.parametrize("log_n", [3, 14, 21])
+@pytest.mark.parametrize("log_n", [3, 14, 21, 22])
 def test_device_matches_host_and_inverse_matches_oracle(Z, ctx, log_n):
     import torch
     n = 1 << log_n
