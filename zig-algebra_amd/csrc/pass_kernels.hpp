@@ -452,6 +452,14 @@ __device__ __forceinline__ void pass_rounds(typename F::E (&x)[N], const PassArg
   }
 }
 
+// Tile of workgroup b in a grid of g.  Workgroups go to the eight XCDs in
+// turn (b to XCD b mod 8), and each XCD has its own L2; with this order the
+// workgroups one XCD runs at a time take consecutive tiles, whose rows of
+// 36-byte elements share cache lines at their edges.  A bijection on [0, g).
+__device__ __forceinline__ size_t xcd_tile(uint32_t b, uint32_t g) {
+  return (g & 7) ? b : (size_t)(b & 7) * (g >> 3) + (b >> 3);
+}
+
 // LB > 0: 2^LB interleaved transforms (see pass_round); a.log_g counts the
 // lane bits, a.L and a.s0 are those of one transform.  PAD: a first pass
 // reads zero at and past a.src_len.
@@ -467,7 +475,7 @@ __global__ __launch_bounds__(512) void ntt_pass_r8_kernel(const PassArgs a) {
   c.lo_bits = a.s0 - 1;
   c.hi_bits = a.L - c.lo_bits - c.K;
   const int lgr = c.lg - LB;  // columns of one transform in the tile
-  const size_t w = blockIdx.x;
+  const size_t w = xcd_tile(blockIdx.x, gridDim.x);
   c.hi = c.lo0 = c.c0 = 0;
   if (a.first) {
     c.c0 = w << lgr;
