@@ -106,6 +106,46 @@ def test_goldilocks_equal_batch_matches_oracle(Z, ctx):
     b.close()
 
 
+@pytest.mark.parametrize("L", [9, 12])
+@pytest.mark.parametrize("power", [1, 3, 5, 7])
+@pytest.mark.parametrize("direction", [0, 1, 2], ids=["fwd", "inv", "coset"])
+def test_goldilocks_radix8_rounds_every_8th_root(Z, ctx, L, power, direction):
+    """Goldilocks 2^9 and 2^12 batches whose transforms share one table run
+    as radix-8 rounds whose 8-point steps multiply by the root's 8th root of
+    unity as a power of two.  The roots w^1, w^3, w^5, w^7 of the standard
+    primitive root w give the four primitive 8th roots (-2^24, -2^72, 2^24,
+    2^72); forward, inverse and coset-forward batches, each transform against
+    the oracle."""
+    F = Z.FIELD_GL
+    count = 6
+    x = Z.fill_uniform(F, 91 + power + L, count << L)
+    t = to_dev(x)
+    w = O.primitive_root(F, L)[1]
+    root = w.copy()
+    for _ in range(power - 1):
+        root = O.field_op(F, "mul", root, w)
+    shift = O.from_int(F, 7)
+    stride = (1 << L) * 8
+    if direction == 2:
+        spec = [(Z.COSET_FORWARD, L, t.data_ptr() + i * stride, root, shift) for i in range(count)]
+    else:
+        spec = [(direction, L, t.data_ptr() + i * stride, root) for i in range(count)]
+    b = Z.Batch(ctx, F, spec)
+    b.execute()
+    b.status()
+    got = to_host(t, x.shape).reshape(count, 1 << L)
+    for i in range(count):
+        row = x[i << L:(i + 1) << L].copy()
+        if direction == 0:
+            assert O.ntt(F, row, L, root) == O.OK
+        elif direction == 1:
+            assert O.intt(F, row, L, root) == O.OK
+        else:
+            assert O.coset_ntt(F, row, L, root, shift) == O.OK
+        assert np.array_equal(got[i], row), (i, L, power, direction)
+    b.close()
+
+
 def test_goldilocks_4096x2p12_against_oracle_digest(Z, ctx):
     F = Z.FIELD_GL
     d = json.load(open(os.path.join(GOLD, "oracle_digests.json")))["goldilocks_batch"]

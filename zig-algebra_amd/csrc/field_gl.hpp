@@ -79,6 +79,36 @@ __device__ __forceinline__ void gl_butterfly(uint64_t &u, uint64_t &v, uint64_t 
   v = d;
 }
 
+// Products by the powers of two that are 8th roots of unity here (2 has
+// order 192 mod p, so the primitive 8th roots are +-2^24 and +-2^72): a
+// shift and one reduction instead of the four-multiply product.  Any u64 x
+// in, canonical out.
+__device__ __forceinline__ uint64_t gl_shl24(uint64_t x) { return gl_reduce128(x << 24, x >> 40); }
+__device__ __forceinline__ uint64_t gl_shl48(uint64_t x) { return gl_reduce128(x << 48, x >> 16); }
+// x 2^72 = y 2^64 with y = x 2^8 (72 bits): (y mod 2^64) 2^64 + (x >> 56) 2^128,
+// and 2^128 = -2^32 (mod p).
+__device__ __forceinline__ uint64_t gl_shl72(uint64_t x) {
+  const uint64_t c = gl_reduce128(0, x << 8);
+  const uint64_t d = (x >> 56) << 32;  // below 2^40
+  uint64_t r = c - d;
+  if (c < d) r -= kGlEps;  // c - d + p
+  return r;
+}
+// 2^72 mod p = 2^8 (2^32 - 1).
+constexpr uint64_t kGl2p72 = 0xFFFFFFFF00ull;
+
+// Butterfly with the product t (< p) already formed: (u, v) <- (u + t, u - t),
+// u any u64, outputs weakly reduced (the carry and borrow fixes of
+// gl_butterfly).
+__device__ __forceinline__ void gl_butterfly_t(uint64_t &u, uint64_t &v, uint64_t t) {
+  uint64_t s = u + t;
+  s += (s < t) ? kGlEps : 0;
+  uint64_t d = u - t;
+  d -= (u < t) ? kGlEps : 0;
+  u = s;
+  v = d;
+}
+
 // Butterfly with twiddle 1 on weakly reduced values.
 __device__ __forceinline__ void gl_butterfly_one(uint64_t &u, uint64_t &v) {
   const uint64_t t = gl_canon(v);
