@@ -107,22 +107,26 @@ def test_goldilocks_equal_batch_matches_oracle(Z, ctx):
 
 
 @pytest.mark.parametrize("L", [9, 12])
-@pytest.mark.parametrize("power", [1, 3, 5, 7])
+@pytest.mark.parametrize("power", [1, 3, 5, 7, 2, 4, 0])
 @pytest.mark.parametrize("direction", [0, 1, 2], ids=["fwd", "inv", "coset"])
 def test_goldilocks_radix8_rounds_every_8th_root(Z, ctx, L, power, direction):
     """Goldilocks 2^9 and 2^12 batches whose transforms share one table run
     as radix-8 rounds whose 8-point steps multiply by the root's 8th root of
     unity as a power of two.  The roots w^1, w^3, w^5, w^7 of the standard
     primitive root w give the four primitive 8th roots (-2^24, -2^72, 2^24,
-    2^72); forward, inverse and coset-forward batches, each transform against
+    2^72).  w^2, w^4 and w^0 = 1 are not primitive (their "8th roots" are a
+    4th root, -1 and 1, which the kernel multiplies by as general products,
+    and their 2^(L-1)-th powers are not -1); the result must still be the
+    radix-2 DAG's over their powers, as the reference computes it for any
+    root.  Forward, inverse and coset-forward batches, each transform against
     the oracle."""
     F = Z.FIELD_GL
     count = 6
     x = Z.fill_uniform(F, 91 + power + L, count << L)
     t = to_dev(x)
     w = O.primitive_root(F, L)[1]
-    root = w.copy()
-    for _ in range(power - 1):
+    root = O.from_int(F, 1)
+    for _ in range(power):
         root = O.field_op(F, "mul", root, w)
     shift = O.from_int(F, 7)
     stride = (1 << L) * 8

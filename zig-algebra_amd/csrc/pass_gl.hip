@@ -215,11 +215,15 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void w
   __shared__ uint32_t s_item;
   uint32_t *P = lds + 2 * (1 << K);
   // P[e] = w^e, e < 2^K: stage K of the flat table holds w^j for j < 2^(K-1)
-  // (at 2^(K-1) - 1), and w^(j + 2^(K-1)) = -w^j.
+  // (at 2^(K-1) - 1), and w^(j + 2^(K-1)) = w^j * w^(2^(K-1)).  The factor
+  // w^(2^(K-1)), formed as w^(2^(K-1)-1) * w from the table, is -1 for a
+  // primitive root; a root of smaller order (the reference accepts any root
+  // and runs the same DAG over its powers) gives 1 or another root of unity.
   constexpr int H = 1 << (K - 1);
+  const uint64_t wH = gl_mul(GlT::load_dev(tw_all, (size_t)(2 * H - 2)), GlT::load_dev(tw_all, (size_t)H));
   for (int e = threadIdx.x; e < (1 << K); e += blockDim.x) {
     uint64_t v = GlT::load_dev(tw_all, (size_t)(H - 1 + (e & (H - 1))));
-    if (e >= H) v = gl_neg(v);
+    if (e >= H) v = gl_mul(v, wH);
     GlT::lds_put(P, 1 << K, e, v);
   }
   __syncthreads();
