@@ -88,6 +88,45 @@ def test_uneven_mixed_batch_matches_oracle(Z, W, ctx, F):
     b.close()
 
 
+@pytest.mark.parametrize("F", [O.FIELD_FR, O.FIELD_GL, O.FIELD_BB, O.FIELD_KB], ids=["fr", "gl", "bb", "kb"])
+def test_batch_roots_of_any_order_follow_the_dag(Z, ctx, F):
+    """The reference runs its DIT DAG over the powers of whatever root it is
+    given (root.zig:65-90 does not check the order).  A batch whose roots are
+    arbitrary field elements or roots of unity of smaller order than the size
+    must give the oracle's result for the same root: whole-transform launches
+    (2^9, 2^12, including Goldilocks 2^12 transforms that share one table),
+    multi-pass level launches, every direction."""
+    import torch
+    arb = O.from_int(F, 123456789)
+    arb2 = O.from_int(F, 987654321)
+    one = O.from_int(F, 1)
+
+    def sq_root(L):  # a root of order 2^(L-1)
+        w = O.primitive_root(F, L)[1]
+        return O.field_op(F, "mul", w, w)
+
+    shift = O.from_int(F, 5)
+    spec = [(0, 12, arb), (0, 12, arb), (0, 12, arb), (1, 12, arb), (2, 12, arb),
+            (0, 9, arb2), (1, 9, sq_root(9)), (2, 9, one),
+            (0, 14, arb), (1, 15, sq_root(15)), (2, 17, arb2), (0, 16, one)]
+    xs = [O.gen_uniform(F, 4100 + i, 1 << L) for i, (d, L, r) in enumerate(spec)]
+    ts = [to_dev(x) for x in xs]
+    items = [(d, L, t.data_ptr(), r, shift if d == 2 else None) for (d, L, r), t in zip(spec, ts)]
+    b = Z.Batch(ctx, F, items)
+    b.execute(stream=torch.cuda.current_stream().cuda_stream)
+    b.status(stream=torch.cuda.current_stream().cuda_stream)
+    for i, ((d, L, r), x, t) in enumerate(zip(spec, xs, ts)):
+        y = x.copy()
+        if d == 0:
+            assert O.ntt(F, y, L, r) == O.OK
+        elif d == 1:
+            assert O.intt(F, y, L, r) == O.OK
+        else:
+            assert O.coset_ntt(F, y, L, r, shift) == O.OK
+        assert np.array_equal(to_host(t, y.shape), y), (i, d, L)
+    b.close()
+
+
 def test_goldilocks_equal_batch_matches_oracle(Z, ctx):
     F = Z.FIELD_GL
     count, L = 64, 12
