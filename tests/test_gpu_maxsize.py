@@ -1,6 +1,7 @@
 """GPU parity at the largest sizes the device path takes (ZNTT_MAX_LOG_N = 28,
 each field's two-adicity bounding its primitive roots): BN254 Fr and
-Goldilocks 2^28, BabyBear 2^27, KoalaBear 2^24.
+Goldilocks 2^28, BabyBear 2^27, KoalaBear 2^24; and the BN254 Fr sizes
+above 2^20 that have no oracle digest (2^21, 2^23, 2^25..2^27).
 
 At these sizes the oracle is too slow to run, so the checks are ones the
 mathematics fixes:
@@ -46,6 +47,22 @@ def to_int(field, words):
 
 @pytest.mark.parametrize("field,L", CASES, ids=[f"f{f}-2^{L}" for f, L in CASES])
 def test_max_size_impulse_and_round_trip(Z, ctx, field, L):
+    impulse_and_round_trip(Z, ctx, field, L)
+
+
+# BN254 Fr sizes between the oracle-digest sizes (2^20, 2^22, 2^24) and 2^28
+# whose pass shapes no other test runs: 2^21 and 2^23 (2^10 tiles, three
+# passes of 7/7/7 and 8/8/7 stages), 2^25..2^27 (2^11 tiles, three passes,
+# the 8- and 9-stage kernels as middle and last passes).
+FR_SHAPES = [21, 23, 25, 26, 27]
+
+
+@pytest.mark.parametrize("L", FR_SHAPES, ids=[f"2^{L}" for L in FR_SHAPES])
+def test_fr_pass_shapes_impulse_and_round_trip(Z, ctx, L):
+    impulse_and_round_trip(Z, ctx, 1, L)
+
+
+def impulse_and_round_trip(Z, ctx, field, L):
     import torch
     n = 1 << L
     W = 4 if field == 1 else 1
