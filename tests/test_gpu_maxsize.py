@@ -1,7 +1,7 @@
 """GPU parity at the largest sizes the device path takes (ZNTT_MAX_LOG_N = 28,
 each field's two-adicity bounding its primitive roots): BN254 Fr and
-Goldilocks 2^28, BabyBear 2^27, KoalaBear 2^24; and the BN254 Fr sizes
-above 2^20 that have no oracle digest (2^21, 2^23, 2^25..2^27).
+Goldilocks 2^28, BabyBear 2^27, KoalaBear 2^24; and the sizes between those
+and the oracle-compared ones whose pass shapes no other test runs.
 
 At these sizes the oracle is too slow to run, so the checks are ones the
 mathematics fixes:
@@ -50,16 +50,18 @@ def test_max_size_impulse_and_round_trip(Z, ctx, field, L):
     impulse_and_round_trip(Z, ctx, field, L)
 
 
-# BN254 Fr sizes between the oracle-digest sizes (2^20, 2^22, 2^24) and 2^28
-# whose pass shapes no other test runs: 2^21 and 2^23 (2^10 tiles, three
+# Sizes between the oracle-compared ones and each field's largest whose pass
+# shapes no other test runs.  BN254 Fr: 2^21 and 2^23 (2^10 tiles, three
 # passes of 7/7/7 and 8/8/7 stages), 2^25..2^27 (2^11 tiles, three passes,
-# the 8- and 9-stage kernels as middle and last passes).
-FR_SHAPES = [21, 23, 25, 26, 27]
+# the 8- and 9-stage kernels as middle and last passes).  The other fields
+# (2^12 tiles, passes of at most 10 stages): 2^23 (8/8/7), 2^25 (9/8/8),
+# 2^26 (9/9/8).
+SHAPES = [(1, 21), (1, 23), (1, 25), (1, 26), (1, 27), (2, 23), (2, 25), (2, 26), (3, 23), (3, 25), (4, 23)]
 
 
-@pytest.mark.parametrize("L", FR_SHAPES, ids=[f"2^{L}" for L in FR_SHAPES])
-def test_fr_pass_shapes_impulse_and_round_trip(Z, ctx, L):
-    impulse_and_round_trip(Z, ctx, 1, L)
+@pytest.mark.parametrize("field,L", SHAPES, ids=[f"f{f}-2^{L}" for f, L in SHAPES])
+def test_pass_shapes_impulse_and_round_trip(Z, ctx, field, L):
+    impulse_and_round_trip(Z, ctx, field, L)
 
 
 def impulse_and_round_trip(Z, ctx, field, L):

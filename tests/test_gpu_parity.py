@@ -79,7 +79,7 @@ def test_device_field_ops_match_oracle(Z, ctx, field):
 # ntt / intt against the oracle at every pass split
 # ---------------------------------------------------------------------------
 
-SIZES = list(range(0, 15)) + [16, 17, 19]
+SIZES = list(range(0, 20))
 
 
 @pytest.mark.parametrize("field", FIELDS, ids=IDS)
@@ -101,6 +101,17 @@ def test_ntt_intt_match_oracle(Z, ctx, field, log_n):
     assert np.array_equal(got_i, exp_i)
     ctx.intt(field, got, log_n, root)
     assert np.array_equal(got, x)
+
+
+# The 32- and 64-bit fields' pass shapes from 2^20 (three passes) that the
+# sizes above do not reach: 2^20 (10/10 stages, the second pass last),
+# 2^21 and 2^22 (7/7/7 and 8/7/7 stages).
+BIG = [(f, L) for f in (O.FIELD_GL, O.FIELD_BB, O.FIELD_KB) for L in (20, 21, 22)]
+
+
+@pytest.mark.parametrize("field,log_n", BIG, ids=[f"{FNAME[f]}-2^{L}" for f, L in BIG])
+def test_three_pass_shapes_match_oracle(Z, ctx, field, log_n):
+    test_ntt_intt_match_oracle(Z, ctx, field, log_n)
 
 
 def test_reference_known_answers_on_device(Z, ctx):
