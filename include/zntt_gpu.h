@@ -295,6 +295,14 @@ int zntt_batch_autotune(zntt_batch *b, int reps, double *best_ms);
  * parity of the first stage), 0 when every pass runs in the generic one. */
 int zntt_batch_shape(const zntt_batch *b, int *tile_log, int *kmax, int *wgs_per_cu, int *order,
                      int *level_kernel);
+/* Sets the batch's shape to one the autotuner could pick (tile_log the
+ * field's tile size or one less: BN254 Fr 11 or 10, the other fields 12 or
+ * 11; kmax tile_log - 2 or tile_log - 3; wgs_per_cu 1..4 within 160 KiB of
+ * LDS per CU; order and level_kernel 0 or 1), e.g. one a previous run's
+ * autotune reported, without timing.  ZNTT_INVALID_ARGUMENT for any other;
+ * on a failure the batch keeps its previous shape.  No counterpart in the
+ * reference, which has no batches. */
+int zntt_batch_set_shape(zntt_batch *b, int tile_log, int kmax, int wgs_per_cu, int order, int level_kernel);
 void zntt_batch_destroy(zntt_batch *b);
 
 /* ------------------------------------------------------------------------ */

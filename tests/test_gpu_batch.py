@@ -230,6 +230,59 @@ def test_large_transforms_batch_matches_oracle_across_shapes(Z, ctx):
     b.close()
 
 
+def all_shapes(F):
+    """Every shape zntt_batch_set_shape accepts (the autotuner's candidates)."""
+    base = 11 if F == O.FIELD_FR else 12
+    words = 9 if F == O.FIELD_FR else (2 if F == O.FIELD_GL else 1)
+    out = []
+    for tl in (base - 1, base):
+        for kmax in (tl - 2, tl - 3):
+            for w in (1, 2, 3, 4):
+                if (words * 4 << tl) * w > 160 * 1024:
+                    continue
+                for order in (0, 1):
+                    for r8 in (0, 1):
+                        out.append((tl, kmax, w, order, r8))
+    return out
+
+
+@pytest.mark.parametrize("F", [O.FIELD_FR, O.FIELD_GL, O.FIELD_BB, O.FIELD_KB], ids=["fr", "gl", "bb", "kb"])
+def test_every_batch_shape_matches_oracle(Z, ctx, F):
+    """Each shape the autotuner may choose (tile size, pass length, resident
+    workgroups, order, level-kernel family) gives every transform the
+    oracle's result: one- to three-pass transforms, every direction.  A shape
+    outside that set is refused and leaves the batch as it was."""
+    import torch
+    spec = [(9, 0), (10, 1), (11, 2), (12, 0), (13, 1), (15, 2), (16, 0), (17, 1), (13, 0), (15, 1)]
+    shift = O.from_int(F, 5)
+    xs = [Z.fill_uniform(F, 3300 + i, 1 << L) for i, (L, d) in enumerate(spec)]
+    expect = [oracle_apply(F, x, L, d, shift) for x, (L, d) in zip(xs, spec)]
+    ts = [to_dev(x) for x in xs]
+    items = [(d, L, t.data_ptr(), Z.primitive_root(F, L), shift if d == 2 else None)
+             for (L, d), t in zip(spec, ts)]
+    b = Z.Batch(ctx, F, items)
+    sp = torch.cuda.current_stream().cuda_stream
+    shapes = all_shapes(F)
+    assert len(shapes) >= 16
+    for sh in shapes:
+        b.set_shape(*sh)
+        assert tuple(b.shape().values()) == sh
+        for t, x in zip(ts, xs):
+            t.copy_(torch.from_numpy(np.ascontiguousarray(x).view(np.int64)))
+        b.execute(stream=sp)
+        b.status(stream=sp)
+        for i, ((L, d), t) in enumerate(zip(spec, ts)):
+            assert np.array_equal(to_host(t, expect[i].shape), expect[i]), (sh, i, L, d)
+    before = b.shape()
+    base = shapes[0][0]
+    for bad in [(base - 2, base - 4, 1, 0, 1), (base, base - 1, 1, 0, 1), (base, base - 2, 5, 0, 1),
+                (base, base - 2, 1, 2, 1), (base, base - 2, 1, 0, 2)]:
+        with pytest.raises(Z.ZnttError):
+            b.set_shape(*bad)
+        assert b.shape() == before
+    b.close()
+
+
 def test_uneven_512_batch_equals_per_call(Z, W, ctx):
     """The full uneven workload (512 transforms, 2^10..2^20, mixed
     directions): batched results equal one plain device call per transform."""

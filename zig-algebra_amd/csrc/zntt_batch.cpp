@@ -35,6 +35,18 @@ Shape default_shape(int field) {
   return field == ZNTT_FIELD_BN254_FR ? Shape{11, 9, 2, 0, 1} : Shape{12, 10, 3, 0, 1};
 }
 
+// The shapes the autotuner may pick: tiles of the field's largest size or
+// half of it, passes two or three stages shorter than the tile, one to four
+// resident workgroups per CU as LDS (160 KiB per CU) allows.
+bool shape_ok(int field, const Shape &sh) {
+  const int base_tl = field == ZNTT_FIELD_BN254_FR ? 11 : 12;
+  if (sh.tile_log != base_tl - 1 && sh.tile_log != base_tl) return false;
+  if (sh.kmax != sh.tile_log - 2 && sh.kmax != sh.tile_log - 3) return false;
+  if (sh.wgs_per_cu < 1 || sh.wgs_per_cu > 4) return false;
+  if (((size_t)zntt::dev_words(field) * 4 << sh.tile_log) * sh.wgs_per_cu > 160 * 1024) return false;
+  return (sh.order == 0 || sh.order == 1) && (sh.r8 == 0 || sh.r8 == 1);
+}
+
 }  // namespace
 
 struct zntt_batch {
@@ -426,10 +438,8 @@ int zntt_batch_autotune(zntt_batch *b, int reps, double *best_ms) {
       for (int kd : {2, 3})
         for (int w : {1, 2, 3, 4})
           for (int order : {0, 1}) {
-            // LDS bounds the resident workgroups: 160 KiB per CU.
-            const size_t lds = (size_t)zntt::dev_words(b->field) * 4 << tl;
-            if (lds * w > 160 * 1024) continue;
-            cands.push_back(Shape{tl, tl - kd, w, order, r8});
+            const Shape sh{tl, tl - kd, w, order, r8};
+            if (shape_ok(b->field, sh)) cands.push_back(sh);
           }
   // Time on shadow copies of the data so the caller's buffers are untouched.
   std::vector<uint32_t *> real(b->xf.size());
@@ -505,6 +515,16 @@ int zntt_batch_shape(const zntt_batch *b, int *tile_log, int *kmax, int *wgs_per
   if (order) *order = b->shape.order;
   if (level_kernel) *level_kernel = b->shape.r8;
   return ZNTT_OK;
+}
+
+int zntt_batch_set_shape(zntt_batch *b, int tile_log, int kmax, int wgs_per_cu, int order, int level_kernel) {
+  if (!b) return ZNTT_INVALID_ARGUMENT;
+  const Shape sh{tile_log, kmax, wgs_per_cu, order, level_kernel};
+  if (!shape_ok(b->field, sh)) return ZNTT_INVALID_ARGUMENT;
+  CallGuard g(b->ctx);
+  const int st = build_plan(b, sh);
+  if (st) build_plan(b, b->shape);  // keep the batch runnable in its previous shape
+  return st;
 }
 
 void zntt_batch_destroy(zntt_batch *b) {

@@ -103,7 +103,7 @@ EXPORTED = [
     "zntt_primitive_root", "zntt_execute_device", "zntt_ctx_synchronize", "zntt_debug_field_op",
     "zntt_probe_peaks", "zntt_fill_uniform", "zntt_ctx_stream",
     "zntt_batch_create", "zntt_batch_execute", "zntt_batch_status", "zntt_batch_autotune", "zntt_batch_shape",
-    "zntt_batch_destroy",
+    "zntt_batch_set_shape", "zntt_batch_destroy",
     "zntt_server_create", "zntt_server_submit", "zntt_server_query", "zntt_server_wait", "zntt_server_cancel",
     "zntt_server_read", "zntt_server_release", "zntt_server_drain", "zntt_server_stats_get", "zntt_server_sim_pump",
     "zntt_server_sim_complete", "zntt_server_destroy", "zntt_fill_counter",
@@ -223,6 +223,7 @@ def _load():
     L.zntt_batch_autotune.argtypes = [vp, i32, ctypes.POINTER(ctypes.c_double)]
     ip = ctypes.POINTER(ctypes.c_int)
     L.zntt_batch_shape.argtypes = [vp, ip, ip, ip, ip, ip]
+    L.zntt_batch_set_shape.argtypes = [vp, i32, i32, i32, i32, i32]
     L.zntt_batch_destroy.argtypes = [vp]
     L.zntt_batch_destroy.restype = None
     L.zntt_server_create.argtypes = [vp, ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)]
@@ -664,6 +665,9 @@ class Batch:
         v = [ctypes.c_int() for _ in range(5)]
         check(lib.zntt_batch_shape(self._h, *[ctypes.byref(x) for x in v]))
         return dict(zip(("tile_log", "kmax", "wgs_per_cu", "order", "level_kernel"), [x.value for x in v]))
+
+    def set_shape(self, tile_log: int, kmax: int, wgs_per_cu: int, order: int, level_kernel: int) -> None:
+        check(lib.zntt_batch_set_shape(self._h, tile_log, kmax, wgs_per_cu, order, level_kernel))
 
     def close(self) -> None:
         if self._h:
