@@ -70,6 +70,32 @@ static void check_butterfly(uint64_t u, uint64_t v, uint64_t w) {
            (unsigned long long)w);
 }
 
+// x * 2^e mod p, canonical, for the shift products of the radix-8 rounds
+// (e = 24, 48, 72: the powers of two that are 8th roots of unity), any u64 x.
+static void check_shifts(uint64_t x) {
+  const uint64_t xm = x % kGlP;
+  const uint64_t w24 = (uint64_t)(((u128)xm << 24) % kGlP);
+  const uint64_t w48 = (uint64_t)(((u128)w24 << 24) % kGlP);
+  const uint64_t w72 = (uint64_t)(((u128)w48 << 24) % kGlP);
+  const uint64_t g24 = zntt::gl_shl24(x), g48 = zntt::gl_shl48(x), g72 = zntt::gl_shl72(x);
+  if ((g24 != w24 || g48 != w48 || g72 != w72) && failures++ < 10)
+    printf("shift x=%016llx got %016llx %016llx %016llx want %016llx %016llx %016llx\n", (unsigned long long)x,
+           (unsigned long long)g24, (unsigned long long)g48, (unsigned long long)g72, (unsigned long long)w24,
+           (unsigned long long)w48, (unsigned long long)w72);
+  // 2^72 mod p as the kernel compares it against the table's 8th root.
+  if (zntt::kGl2p72 != (uint64_t)(((u128)1 << 72) % kGlP) && failures++ < 10) printf("kGl2p72 wrong\n");
+}
+
+// gl_butterfly_t: u any u64, t < p; outputs congruent to u + t and u - t.
+static void check_butterfly_t(uint64_t u, uint64_t t) {
+  uint64_t s = u, d = 0;
+  zntt::gl_butterfly_t(s, d, t);
+  const uint64_t um = u % kGlP;
+  if ((s % kGlP != (uint64_t)(((u128)um + t) % kGlP) || d % kGlP != (uint64_t)(((u128)um + kGlP - t) % kGlP)) &&
+      failures++ < 10)
+    printf("butterfly_t u=%016llx t=%016llx\n", (unsigned long long)u, (unsigned long long)t);
+}
+
 int main(int argc, char **argv) {
   const long draws = argc > 1 ? atol(argv[1]) : 4000000;
   const uint64_t edges[] = {0,
@@ -93,11 +119,16 @@ int main(int argc, char **argv) {
     for (int j = 0; j < ne; j++) {
       check_reduce(edges[i], edges[j]);
       check_mul(edges[i], edges[j]);
+      if (edges[j] < kGlP) check_butterfly_t(edges[i], edges[j]);
       for (int k = 0; k < ne; k++)
         if (edges[k] < kGlP) check_butterfly(edges[i], edges[j], edges[k]);
     }
+  for (int i = 0; i < ne; i++) check_shifts(edges[i]);
+  for (int e = 0; e < 64; e++) check_shifts((uint64_t)1 << e), check_shifts(~((uint64_t)1 << e));
   for (long it = 0; it < draws; it++) {
     const uint64_t a = next64(), b = next64(), c = next64();
+    check_shifts(a);
+    check_butterfly_t(a, b % kGlP);
     // Random 128-bit values and random products.
     check_reduce(a, b);
     check_mul(a, b);

@@ -2,7 +2,8 @@
 
 tests/hostsim/gl_reduce_main.cpp includes the same header the kernels use,
 built by g++ as host functions, and compares gl_reduce128, gl_mul,
-gl_butterfly and gl_butterfly_one with unsigned __int128 arithmetic mod p:
+gl_butterfly, gl_butterfly_one, gl_butterfly_t and the radix-8 rounds' shift
+products gl_shl24/48/72 with unsigned __int128 arithmetic mod p:
 every pair of edge values (0, 1, eps, p - 1, p, 2^64 - 1, ...), random
 128-bit values and products, and 128-bit values built to take each of the
 reduction's three cases (carried; not carried with r >= eps; not carried with
