@@ -201,6 +201,23 @@ def test_large_requests_run_alone_and_match_oracle(Z, ctx):
     srv.close()
 
 
+@pytest.mark.parametrize("field", ALL[1:], ids=IDS[1:])
+def test_large_requests_of_the_small_fields_match_oracle(Z, ctx, field):
+    """Goldilocks, BabyBear and KoalaBear requests of 2^18 .. 2^22
+    evaluations (two and three passes, the zero-padding first pass with one
+    to three padded stages) served together with small ones: each equals the
+    oracle's LDE."""
+    shifts = [O.from_int(field, 5), O.from_int(field, 7)]
+    reqs = [(18, 3, 0, 701), (19, 2, 1, 702), (17, 1, 0, 703), (20, 2, 1, 704), (11, 3, 0, 705), (9, 2, 1, 706)]
+    srv = Z.Server(ctx, field, hbm_budget=1 << 31, max_log_n=22)
+    ts = [srv.submit(k, b, shifts[si], seed=4, stream_id=sid, keep=True) for k, b, si, sid in reqs]
+    srv.drain()
+    for (k, b, si, sid), t in zip(reqs, ts):
+        assert np.array_equal(srv.read(t), expected(field, 4, sid, k, b, shifts[si])), (k, b, si, sid)
+        srv.release(t)
+    srv.close()
+
+
 def test_config5_size_requests_match_oracle_digests(Z, ctx):
     """Requests of the serving benchmark's own schedule at its largest sizes
     (k 21 and 22, blowup 4 and 8: 2^23 .. 2^25 evaluations, the zero-padding
