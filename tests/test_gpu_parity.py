@@ -274,7 +274,7 @@ def test_bit_reverse_matches_oracle(Z, ctx, field):
 @pytest.mark.parametrize("field", FIELDS, ids=IDS)
 def test_coset_and_lde_match_oracle_and_horner(Z, ctx, field):
     shift = O.from_int(field, GENERATOR[field])
-    for log_n in (0, 1, 6, 12, 15):
+    for log_n in (0, 1, 6, 12, 15, 17, 19, 20):
         x = O.gen_uniform(field, 300 + log_n, 1 << log_n)
         root = O.primitive_root(field, log_n)[1]
         exp = x.copy()
@@ -289,7 +289,7 @@ def test_coset_and_lde_match_oracle_and_horner(Z, ctx, field):
         coeffs = O.from_ints(field, [int(v, 16) for v in c["coeffs"]])
         out = ctx.lde(field, coeffs, c["log_k"], c["log_blowup"], O.from_int(field, int(c["coset_shift"], 16)))
         assert [O.value(field, out[i]) for i in range(out.shape[0])] == [int(v, 16) for v in c["evals"]]
-    for log_k, log_b in ((10, 2), (12, 3)):
+    for log_k, log_b in ((10, 2), (12, 3), (15, 2), (17, 3)):
         coeffs = O.gen_uniform(field, 40 + log_k, 1 << log_k)
         st, exp = O.lde(field, coeffs, log_k, log_b, shift)
         assert st == O.OK
@@ -450,7 +450,7 @@ def test_twiddled_device_matches_oracle(Z, ctx, field):
     table and over a foreign (non-power) one."""
     import torch
     sp = torch.cuda.current_stream().cuda_stream
-    for log_n in (9, 13, 17):
+    for log_n in (9, 13, 17, 20):
         say("twiddled device", FNAME[field], log_n)
         root = O.primitive_root(field, log_n)[1]
         reg = ctx.precompute_twiddles(field, log_n, root)
