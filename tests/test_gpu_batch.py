@@ -106,9 +106,10 @@ def test_batch_roots_of_any_order_follow_the_dag(Z, ctx, F):
         return O.field_op(F, "mul", w, w)
 
     shift = O.from_int(F, 5)
+    zero = O.from_int(F, 0)
     spec = [(0, 12, arb), (0, 12, arb), (0, 12, arb), (1, 12, arb), (2, 12, arb),
-            (0, 9, arb2), (1, 9, sq_root(9)), (2, 9, one),
-            (0, 14, arb), (1, 15, sq_root(15)), (2, 17, arb2), (0, 16, one)]
+            (0, 9, arb2), (1, 9, sq_root(9)), (2, 9, one), (0, 10, zero), (2, 11, zero),
+            (0, 14, arb), (1, 15, sq_root(15)), (2, 17, arb2), (0, 16, one), (0, 13, zero)]
     xs = [O.gen_uniform(F, 4100 + i, 1 << L) for i, (d, L, r) in enumerate(spec)]
     ts = [to_dev(x) for x in xs]
     items = [(d, L, t.data_ptr(), r, shift if d == 2 else None) for (d, L, r), t in zip(spec, ts)]

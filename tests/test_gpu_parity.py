@@ -147,14 +147,20 @@ def test_fr_dft_vectors_on_device(Z, ctx):
 def test_non_primitive_root_follows_the_dag(Z, ctx, field):
     """For a root that is not a primitive root the reference's DIT DAG is the
     only definition; the device must reproduce it exactly."""
-    for log_n in (5, 12, 15):
+    for log_n in (5, 12, 15, 20):
         x = O.gen_uniform(field, 77 + log_n, 1 << log_n)
-        root = O.from_int(field, 123456789)
-        exp = x.copy()
-        O.ntt(field, exp, log_n, root)
-        got = x.copy()
-        ctx.ntt(field, got, log_n, root)
-        assert np.array_equal(got, exp), log_n
+        for r in (123456789, 0, 1):  # any element; 0 and 1 make every twiddle past j = 0 zero or one
+            root = O.from_int(field, r)
+            exp = x.copy()
+            O.ntt(field, exp, log_n, root)
+            got = x.copy()
+            ctx.ntt(field, got, log_n, root)
+            assert np.array_equal(got, exp), (log_n, r)
+            exp = x.copy()
+            O.intt(field, exp, log_n, root)  # inv(0) = 0, as the reference returns (field.zig:384-386, montgomery.zig:386-388)
+            got = x.copy()
+            ctx.intt(field, got, log_n, root)
+            assert np.array_equal(got, exp), (log_n, r, "intt")
 
 
 # ---------------------------------------------------------------------------
