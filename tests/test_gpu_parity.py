@@ -303,7 +303,43 @@ def test_coset_and_lde_match_oracle_and_horner(Z, ctx, field):
         assert np.array_equal(got, exp)
 
 
+@pytest.mark.parametrize("field", FIELDS, ids=IDS)
+def test_lde_and_coset_edge_cases(Z, ctx, field):
+    """Degenerate LDE and coset shapes against the oracle: one coefficient,
+    no blowup, blowup of a single coefficient, shifts 0 and 1; and the size
+    errors of zntt_lde raised before its buffers are touched (sizes whose
+    buffers could not be allocated, so the C ABI is called directly)."""
+    import ctypes
+    for log_k, log_b in ((0, 0), (0, 3), (4, 0), (1, 1), (3, 2)):
+        coeffs = O.gen_uniform(field, 500 + 8 * log_k + log_b, 1 << log_k)
+        for sh in (GENERATOR[field], 0, 1):
+            shift = O.from_int(field, sh)
+            st, exp = O.lde(field, coeffs, log_k, log_b, shift)
+            assert st == O.OK
+            got = ctx.lde(field, coeffs, log_k, log_b, shift)
+            assert np.array_equal(got, exp), (log_k, log_b, sh)
+    for log_n in (0, 1, 3, 9):
+        x = O.gen_uniform(field, 520 + log_n, 1 << log_n)
+        root = O.primitive_root(field, log_n)[1]
+        for sh in (0, 1):
+            shift = O.from_int(field, sh)
+            exp = x.copy()
+            assert O.coset_ntt(field, exp, log_n, root, shift) == O.OK
+            got = x.copy()
+            ctx.coset_ntt(field, got, log_n, root, shift)
+            assert np.array_equal(got, exp), (log_n, sh)
+    coeffs = O.gen_uniform(field, 7, 1)
+    out = O.empty(field, 1)
+    shift = O.from_int(field, 1)
+    too_deep = {O.FIELD_FR: 29, O.FIELD_GL: 33, O.FIELD_BB: 28, O.FIELD_KB: 25}[field]  # two-adicity + 1
+    for log_k, log_b, err in ((0, too_deep, Z.OrderTooLarge), (60, 10, Z.LogTooLarge), (70, 0, Z.LogTooLarge)):
+        st = Z.lib.zntt_lde(ctx._h, field, coeffs.ctypes.data, log_k, log_b, shift.ctypes.data, out.ctypes.data)
+        with pytest.raises(err):
+            Z.check(st)
+
+
 # ---------------------------------------------------------------------------
+# Full-size workloads (BN254 Fr 2^16 .. 2^24) against committed digests# ---------------------------------------------------------------------------
 # Full-size workloads (BN254 Fr 2^16 .. 2^24) against committed digests
 # ---------------------------------------------------------------------------
 
