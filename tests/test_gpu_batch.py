@@ -251,10 +251,11 @@ def all_shapes(F):
 def test_every_batch_shape_matches_oracle(Z, ctx, F):
     """Each shape the autotuner may choose (tile size, pass length, resident
     workgroups, order, level-kernel family) gives every transform the
-    oracle's result: one- to three-pass transforms, every direction.  A shape
+    oracle's result: one- to three-pass transforms from 2^3, every direction.  A shape
     outside that set is refused and leaves the batch as it was."""
     import torch
-    spec = [(9, 0), (10, 1), (11, 2), (12, 0), (13, 1), (15, 2), (16, 0), (17, 1), (13, 0), (15, 1)]
+    spec = [(9, 0), (10, 1), (11, 2), (12, 0), (13, 1), (15, 2), (16, 0), (17, 1), (13, 0), (15, 1),
+            (3, 0), (4, 2), (6, 1), (8, 0)]
     shift = O.from_int(F, 5)
     xs = [Z.fill_uniform(F, 3300 + i, 1 << L) for i, (L, d) in enumerate(spec)]
     expect = [oracle_apply(F, x, L, d, shift) for x, (L, d) in zip(xs, spec)]
