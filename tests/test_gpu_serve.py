@@ -218,6 +218,34 @@ def test_large_requests_of_the_small_fields_match_oracle(Z, ctx, field):
     srv.close()
 
 
+@pytest.mark.parametrize("field", ALL, ids=IDS)
+def test_smallest_requests_and_request_errors(Z, ctx, field):
+    """The smallest LDEs the server takes (2^3 evaluations from one, two or
+    four coefficients, and one coefficient blown up 16-fold) against the
+    oracle, served together; and the request errors, raised at submit:
+    fewer than 2^3 evaluations, a coefficient count that is not 2^log_n,
+    sizes past the field's two-adicity or 2^64."""
+    shift = O.from_int(field, 5)
+    reqs = [(0, 3, 801), (1, 2, 802), (2, 1, 803), (0, 4, 804), (3, 0, 805)]
+    srv = Z.Server(ctx, field, hbm_budget=1 << 28, max_log_n=12)
+    ts = [srv.submit(k, b, shift, seed=4, stream_id=sid, keep=True) for k, b, sid in reqs]
+    srv.drain()
+    for (k, b, sid), t in zip(reqs, ts):
+        assert np.array_equal(srv.read(t), expected(field, 4, sid, k, b, shift)), (k, b)
+        srv.release(t)
+    with pytest.raises(Z.Unsupported):
+        srv.submit(1, 1, shift, seed=4)
+    with pytest.raises(Z.LengthMismatch):
+        srv.submit(4, 2, shift, seed=4, length=15)
+    deep = {O.FIELD_FR: 29, O.FIELD_GL: 33, O.FIELD_BB: 28, O.FIELD_KB: 25}[field]
+    with pytest.raises(Z.OrderTooLarge):
+        srv.submit(0, deep, shift, seed=4, length=1)
+    with pytest.raises(Z.LogTooLarge):
+        srv.submit(60, 4, shift, seed=4, length=1)
+    assert srv.stats()["bytes_in_use"] == 0
+    srv.close()
+
+
 def test_config5_size_requests_match_oracle_digests(Z, ctx):
     """Requests of the serving benchmark's own schedule at its largest sizes
     (k 21 and 22, blowup 4 and 8: 2^23 .. 2^25 evaluations, the zero-padding
