@@ -25,6 +25,10 @@ Two more scenarios use the stand-in runtime's call counters:
 * after zntt_ctx_reserve the seven zig-ntt functions allocate nothing at the
   reserved sizes (every field, new roots, registered and foreign tables,
   across a trim), so device memory can run out only at the reservation.
+
+And one the stand-in runtime's memory makes possible: host buffers of 32 and
+128 MiB and a 64 MiB LDE input go through the context's page-locked ring in
+chunks and come back intact.
 """
 import os
 import shutil
@@ -37,7 +41,7 @@ SIM = os.path.join(HERE, "hostsim")
 OUT = os.path.join(SIM, "_build")
 SCENARIOS = ["batch_outlives_ctx", "trim_with_live_batch", "server_outlives_ctx", "trim_with_live_server",
              "late_destroy_after_new_ctx", "noalloc_allocates_nothing", "streams_share_scratch_in_order",
-             "reserved_calls_allocate_nothing"]
+             "reserved_calls_allocate_nothing", "host_copies_through_the_ring"]
 
 
 @pytest.fixture(scope="module")

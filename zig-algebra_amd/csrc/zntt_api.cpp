@@ -250,6 +250,7 @@ void release_cache(zntt_ctx *c, bool all = false) {
   if (c->pinned) hipHostFree(c->pinned);
   c->pinned = nullptr;
   c->pinned_bytes = 0;
+  if (all || c->resv_log_n < 0) ring_free(c);
 }
 
 void ctx_retain(zntt_ctx *c) { c->refs.fetch_add(1, std::memory_order_relaxed); }
@@ -497,16 +498,13 @@ int with_staging(zntt_ctx *c, int field, void *data, size_t n, Fn &&fn) {
   const size_t bytes = n * user_bytes(field);
   int st = ensure(c, &c->staging, &c->staging_bytes, bytes);
   if (st) return st;
-  hipError_t e = hipMemcpyAsync(c->staging, data, bytes, hipMemcpyHostToDevice, c->stream);
-  if (e != hipSuccess) return hip_status(e);
-  st = fn(c->staging);
+  st = copy_to_device(c, c->staging, data, bytes);
+  if (!st) st = fn(c->staging);
   if (st) {
     hipStreamSynchronize(c->stream);
     return st;
   }
-  e = hipMemcpyAsync(data, c->staging, bytes, hipMemcpyDeviceToHost, c->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-  return hip_status(e);
+  return copy_to_host(c, data, c->staging, bytes);
 }
 
 // Fingerprint of a host twiddle table (stage s holds 2^s elements): a hash
@@ -752,6 +750,8 @@ int zntt_ctx_reserve(zntt_ctx *c, size_t max_log_n) {
     st = dev_alloc(c, consts, (void **)&c->resv_consts);
     if (!st) c->resv_consts_bytes = consts;
   }
+  // The page-locked ring of large host-buffer copies.
+  if (!st && ring_wanted(n * ub)) st = ring_ensure(c);
   for (ResvSlot &sl : c->resv) {
     if (st || sl.bytes >= n * db) continue;
     uint32_t *t = nullptr;
@@ -1093,8 +1093,9 @@ int zntt_lde(zntt_ctx *c, int field, const void *coeffs, size_t log_k, size_t lo
   st = ensure(c, &c->staging, &c->staging_bytes, n * ub);
   if (st) return st;
   hipError_t e = hipMemsetAsync((uint8_t *)c->staging + k * ub, 0, (n - k) * ub, c->stream);
-  if (e == hipSuccess) e = hipMemcpyAsync(c->staging, coeffs, k * ub, hipMemcpyHostToDevice, c->stream);
   if (e != hipSuccess) return hip_status(e);
+  st = copy_to_device(c, c->staging, coeffs, k * ub);
+  if (st) return st;
   if (L == 0) {
     // f(shift) with a single coefficient is the coefficient itself.
     memcpy(out, coeffs, ub);
@@ -1102,9 +1103,7 @@ int zntt_lde(zntt_ctx *c, int field, const void *coeffs, size_t log_k, size_t lo
   }
   st = run_device(c, field, (int)L, c->staging, t->table, cos, nullptr, c->stream);
   if (st) return st;
-  e = hipMemcpyAsync(out, c->staging, n * ub, hipMemcpyDeviceToHost, c->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-  return hip_status(e);
+  return copy_to_host(c, out, c->staging, n * ub);
 }
 
 int zntt_poly_mul(zntt_ctx *c, int field, const void *a, size_t a_len, const void *b, size_t b_len, void *out,
