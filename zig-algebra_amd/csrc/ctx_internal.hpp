@@ -159,16 +159,16 @@ int dev_alloc(zntt_ctx *c, size_t bytes, void **out);
 int ensure(zntt_ctx *c, void **buf, size_t *have, size_t bytes);
 // Grows the context's page-locked host buffer to at least `bytes`.
 int ensure_pinned(zntt_ctx *c, size_t bytes);
-// Host-buffer transfers on the context's stream (host_copy.cpp).  Large ones
-// go through the context's ring of page-locked slots, copied by a few
-// threads while the DMA engine moves the previous slot.  copy_to_device
-// returns once the host buffer has been read (the uploads may still be in
-// flight); copy_to_host returns with the stream synchronised.
+// Host-buffer transfers on `stream` (host_copy.cpp).  Large ones go through
+// the context's ring of page-locked slots, copied by a few threads while the
+// DMA engine moves the previous slot.  copy_to_device returns once the host
+// buffer has been read (the uploads may still be in flight); copy_to_host
+// returns with the stream synchronised.  Caller holds the context lock.
 bool ring_wanted(size_t bytes);
 int ring_ensure(zntt_ctx *c);
 void ring_free(zntt_ctx *c);
-int copy_to_device(zntt_ctx *c, void *dev, const void *host, size_t bytes);
-int copy_to_host(zntt_ctx *c, void *host, const void *dev, size_t bytes);
+int copy_to_device(zntt_ctx *c, void *dev, const void *host, size_t bytes, hipStream_t stream);
+int copy_to_host(zntt_ctx *c, void *host, const void *dev, size_t bytes, hipStream_t stream);
 int power_table(zntt_ctx *c, int field, const Elem &root, int L, hipStream_t s, TwiddleEntry **out);
 int coset_table(zntt_ctx *c, int field, const Elem &shift, size_t count, hipStream_t s, const uint32_t **out);
 // One transform of 2^L user-layout values on the device (tw: table of the

@@ -95,8 +95,22 @@ class CopyPool {
 struct HostRing {
   uint8_t *buf = nullptr;  // kSlots * kChunk bytes, page-locked
   hipEvent_t ev[kSlots] = {};
+  bool busy[kSlots] = {};  // ev[s] follows a transfer of slot s not yet waited for
   CopyPool pool;
   uint8_t *slot(int s) { return buf + (size_t)s * kChunk; }
+  // Waits until no transfer uses slot s, from this copy or an earlier one
+  // (a call may run several copies before it synchronises its stream).
+  hipError_t claim(int s) {
+    if (!busy[s]) return hipSuccess;
+    const hipError_t e = hipEventSynchronize(ev[s]);
+    if (e == hipSuccess) busy[s] = false;
+    return e;
+  }
+  hipError_t mark(int s, hipStream_t stream) {
+    const hipError_t e = hipEventRecord(ev[s], stream);
+    busy[s] = e == hipSuccess;
+    return e;
+  }
 };
 
 bool ring_wanted(size_t bytes) { return bytes > kChunk; }
@@ -128,9 +142,9 @@ void ring_free(zntt_ctx *c) {
   c->ring = nullptr;
 }
 
-int copy_to_device(zntt_ctx *c, void *dev, const void *host, size_t bytes) {
+int copy_to_device(zntt_ctx *c, void *dev, const void *host, size_t bytes, hipStream_t stream) {
   if (!ring_wanted(bytes) || ring_ensure(c) != ZNTT_OK)
-    return hip_status(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, c->stream));
+    return hip_status(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, stream));
   HostRing *r = c->ring;
   const uint8_t *h = (const uint8_t *)host;
   uint8_t *d = (uint8_t *)dev;
@@ -138,19 +152,19 @@ int copy_to_device(zntt_ctx *c, void *dev, const void *host, size_t bytes) {
   for (size_t off = 0, i = 0; off < bytes && e == hipSuccess; off += kChunk, i++) {
     const int s = (int)(i % kSlots);
     const size_t len = std::min(kChunk, bytes - off);
-    if (i >= (size_t)kSlots) e = hipEventSynchronize(r->ev[s]);  // the slot's last upload is done
+    e = r->claim(s);
     if (e != hipSuccess) break;
     r->pool.copy(r->slot(s), h + off, len);
-    e = hipMemcpyAsync(d + off, r->slot(s), len, hipMemcpyHostToDevice, c->stream);
-    if (e == hipSuccess) e = hipEventRecord(r->ev[s], c->stream);
+    e = hipMemcpyAsync(d + off, r->slot(s), len, hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess) e = r->mark(s, stream);
   }
   return hip_status(e);
 }
 
-int copy_to_host(zntt_ctx *c, void *host, const void *dev, size_t bytes) {
+int copy_to_host(zntt_ctx *c, void *host, const void *dev, size_t bytes, hipStream_t stream) {
   if (!ring_wanted(bytes) || ring_ensure(c) != ZNTT_OK) {
-    hipError_t e = hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    hipError_t e = hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
     return hip_status(e);
   }
   HostRing *r = c->ring;
@@ -162,8 +176,9 @@ int copy_to_host(zntt_ctx *c, void *host, const void *dev, size_t bytes) {
   auto issue = [&](size_t i) {
     const int s = (int)(i % kSlots);
     const size_t off = i * kChunk, len = std::min(kChunk, bytes - off);
-    hipError_t e = hipMemcpyAsync(r->slot(s), d + off, len, hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess) e = hipEventRecord(r->ev[s], c->stream);
+    hipError_t e = r->claim(s);
+    if (e == hipSuccess) e = hipMemcpyAsync(r->slot(s), d + off, len, hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = r->mark(s, stream);
     return e;
   };
   hipError_t e = hipSuccess;
@@ -171,12 +186,12 @@ int copy_to_host(zntt_ctx *c, void *host, const void *dev, size_t bytes) {
   for (size_t i = 0; i < chunks && e == hipSuccess; i++) {
     const int s = (int)(i % kSlots);
     const size_t off = i * kChunk, len = std::min(kChunk, bytes - off);
-    e = hipEventSynchronize(r->ev[s]);
+    e = r->claim(s);  // the slot's download has landed
     if (e != hipSuccess) break;
     r->pool.copy(h + off, r->slot(s), len);
     if (i + kSlots < chunks) e = issue(i + kSlots);
   }
-  hipError_t e2 = hipStreamSynchronize(c->stream);
+  hipError_t e2 = hipStreamSynchronize(stream);
   return hip_status(e != hipSuccess ? e : e2);
 }
 

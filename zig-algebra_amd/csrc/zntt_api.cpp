@@ -498,13 +498,13 @@ int with_staging(zntt_ctx *c, int field, void *data, size_t n, Fn &&fn) {
   const size_t bytes = n * user_bytes(field);
   int st = ensure(c, &c->staging, &c->staging_bytes, bytes);
   if (st) return st;
-  st = copy_to_device(c, c->staging, data, bytes);
+  st = copy_to_device(c, c->staging, data, bytes, c->stream);
   if (!st) st = fn(c->staging);
   if (st) {
     hipStreamSynchronize(c->stream);
     return st;
   }
-  return copy_to_host(c, data, c->staging, bytes);
+  return copy_to_host(c, data, c->staging, bytes, c->stream);
 }
 
 // Fingerprint of a host twiddle table (stage s holds 2^s elements): a hash
@@ -582,8 +582,9 @@ int upload_foreign(zntt_ctx *c, int field, size_t log_n, const void *const *stag
   for (size_t k = 0; k < log_n; k++) memcpy(host.data() + (((size_t)1 << k) - 1) * ub, stages[k], ((size_t)1 << k) * ub);
   const Elem one = f_one(field);
   memcpy(host.data() + (n - 1) * ub, one.w, ub);
-  hipError_t e = hipMemcpyAsync(user, host.data(), n * ub, hipMemcpyHostToDevice, s);
-  if (e == hipSuccess) e = zntt::launch_table_from_user(field, n, dev, (const uint32_t *)user, s);
+  st = copy_to_device(c, user, host.data(), n * ub, s);
+  if (st) return st;
+  hipError_t e = zntt::launch_table_from_user(field, n, dev, (const uint32_t *)user, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);  // `host` goes out of scope
   *out = dev;
   return hip_status(e);
@@ -908,11 +909,12 @@ int zntt_precompute_twiddles(zntt_ctx *c, int field, size_t log_n, const void *r
                         hipMemcpyDeviceToDevice, c->stream);
   if (he == hipSuccess) he = zntt::launch_table_to_user(field, n - 1, (uint32_t *)c->staging, e.table, c->stream);
   std::vector<uint8_t> host((n - 1) * ub);
-  if (he == hipSuccess) he = hipMemcpyAsync(host.data(), c->staging, (n - 1) * ub, hipMemcpyDeviceToHost, c->stream);
-  if (he == hipSuccess) he = hipStreamSynchronize(c->stream);
-  if (he != hipSuccess) {
+  int cst = hip_status(he);
+  if (!cst) cst = copy_to_host(c, host.data(), c->staging, (n - 1) * ub, c->stream);
+  if (cst) {
+    hipStreamSynchronize(c->stream);
     dev_free(c, e.table, e.bytes);
-    return hip_status(he);
+    return cst;
   }
   for (size_t s = 0; s < log_n; s++) memcpy(stages[s], host.data() + (((size_t)1 << s) - 1) * ub, ((size_t)1 << s) * ub);
   e.key.assign(stages, stages + log_n);
@@ -1094,7 +1096,7 @@ int zntt_lde(zntt_ctx *c, int field, const void *coeffs, size_t log_k, size_t lo
   if (st) return st;
   hipError_t e = hipMemsetAsync((uint8_t *)c->staging + k * ub, 0, (n - k) * ub, c->stream);
   if (e != hipSuccess) return hip_status(e);
-  st = copy_to_device(c, c->staging, coeffs, k * ub);
+  st = copy_to_device(c, c->staging, coeffs, k * ub, c->stream);
   if (st) return st;
   if (L == 0) {
     // f(shift) with a single coefficient is the coefficient itself.
@@ -1103,7 +1105,7 @@ int zntt_lde(zntt_ctx *c, int field, const void *coeffs, size_t log_k, size_t lo
   }
   st = run_device(c, field, (int)L, c->staging, t->table, cos, nullptr, c->stream);
   if (st) return st;
-  return copy_to_host(c, out, c->staging, n * ub);
+  return copy_to_host(c, out, c->staging, n * ub, c->stream);
 }
 
 int zntt_poly_mul(zntt_ctx *c, int field, const void *a, size_t a_len, const void *b, size_t b_len, void *out,
@@ -1122,14 +1124,14 @@ int zntt_poly_mul(zntt_ctx *c, int field, const void *a, size_t a_len, const voi
   if (st) return st;
   dev = c->aux;
   uint8_t *da = (uint8_t *)dev, *db = da + na * ub, *dout = db + nb * ub;
-  hipError_t e = hipMemcpyAsync(da, a, na * ub, hipMemcpyHostToDevice, c->stream);
-  if (e == hipSuccess) e = hipMemcpyAsync(db, b, nb * ub, hipMemcpyHostToDevice, c->stream);
-  if (e != hipSuccess) return hip_status(e);
-  st = poly_mul_device(c, field, da, na, db, nb, dout, c->stream);
-  if (st) return st;
-  e = hipMemcpyAsync(out, dout, m * ub, hipMemcpyDeviceToHost, c->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-  if (e != hipSuccess) return hip_status(e);
+  st = copy_to_device(c, da, a, na * ub, c->stream);
+  if (!st) st = copy_to_device(c, db, b, nb * ub, c->stream);
+  if (!st) st = poly_mul_device(c, field, da, na, db, nb, dout, c->stream);
+  if (!st) st = copy_to_host(c, out, dout, m * ub, c->stream);
+  if (st) {
+    hipStreamSynchronize(c->stream);
+    return st;
+  }
   *out_len = m;
   return ZNTT_OK;
 }

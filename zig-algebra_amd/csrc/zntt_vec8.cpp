@@ -97,12 +97,13 @@ int vec8_host(zntt_ctx *c, bool inverse, void *data, size_t len, size_t log_n, c
   memcpy(&rv, root, 8);
   CallGuard g(c, c->stream);
   if ((st = ensure(c, &c->staging, &c->staging_bytes, len * 8))) return st;
-  hipError_t e = hipMemcpyAsync(c->staging, data, len * 8, hipMemcpyHostToDevice, c->stream);
-  if (e != hipSuccess) return hip_status(e);
-  if ((st = vec8_device(c, inverse, c->staging, (int)log_n, rv, c->stream))) return st;
-  e = hipMemcpyAsync(data, c->staging, len * 8, hipMemcpyDeviceToHost, c->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-  return hip_status(e);
+  st = copy_to_device(c, c->staging, data, len * 8, c->stream);
+  if (!st) st = vec8_device(c, inverse, c->staging, (int)log_n, rv, c->stream);
+  if (st) {
+    hipStreamSynchronize(c->stream);
+    return st;
+  }
+  return copy_to_host(c, data, c->staging, len * 8, c->stream);
 }
 
 }  // namespace
