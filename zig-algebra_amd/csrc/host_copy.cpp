@@ -117,7 +117,12 @@ bool ring_wanted(size_t bytes) { return bytes > kChunk; }
 
 int ring_ensure(zntt_ctx *c) {
   if (c->ring) return ZNTT_OK;
-  auto *r = new HostRing();
+  HostRing *r = nullptr;
+  try {
+    r = new HostRing();  // starts the copy threads
+  } catch (...) {
+    return ZNTT_OUT_OF_MEMORY;  // no thread or memory for them: the plain copy serves
+  }
   hipError_t e = hipHostMalloc((void **)&r->buf, (size_t)kSlots * kChunk, hipHostMallocDefault);
   for (int s = 0; s < kSlots && e == hipSuccess; s++) e = hipEventCreateWithFlags(&r->ev[s], hipEventDisableTiming);
   if (e != hipSuccess) {
