@@ -87,6 +87,7 @@ struct zntt_ctx {
   void *pinned = nullptr;  // page-locked host staging (small read-backs and uploads)
   size_t pinned_bytes = 0;
   zntt_detail::HostRing *ring = nullptr;  // large host-buffer copies (host_copy.cpp)
+  bool ring_failed = false;               // its setup failed: plain copies until the next trim
   // The eight-lane M31 transform's twiddle table for (m31_root, m31_log_n);
   // kept so that repeated calls with one root skip the rebuild.
   void *m31_table = nullptr;

@@ -117,11 +117,13 @@ bool ring_wanted(size_t bytes) { return bytes > kChunk; }
 
 int ring_ensure(zntt_ctx *c) {
   if (c->ring) return ZNTT_OK;
+  if (c->ring_failed) return ZNTT_OUT_OF_MEMORY;
   HostRing *r = nullptr;
   try {
     r = new HostRing();  // starts the copy threads
   } catch (...) {
-    return ZNTT_OUT_OF_MEMORY;  // no thread or memory for them: the plain copy serves
+    c->ring_failed = true;  // no thread or memory for them: the plain copy serves
+    return ZNTT_OUT_OF_MEMORY;
   }
   hipError_t e = hipHostMalloc((void **)&r->buf, (size_t)kSlots * kChunk, hipHostMallocDefault);
   for (int s = 0; s < kSlots && e == hipSuccess; s++) e = hipEventCreateWithFlags(&r->ev[s], hipEventDisableTiming);
@@ -131,6 +133,7 @@ int ring_ensure(zntt_ctx *c) {
       if (ev) hipEventDestroy(ev);
     if (r->buf) hipHostFree(r->buf);
     delete r;
+    c->ring_failed = true;
     return hip_status(e);
   }
   c->ring = r;
@@ -139,6 +142,7 @@ int ring_ensure(zntt_ctx *c) {
 
 // The caller has synchronised the context's stream (no copy uses the ring).
 void ring_free(zntt_ctx *c) {
+  c->ring_failed = false;
   HostRing *r = c->ring;
   if (!r) return;
   for (auto &ev : r->ev) hipEventDestroy(ev);

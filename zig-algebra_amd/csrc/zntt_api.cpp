@@ -751,8 +751,9 @@ int zntt_ctx_reserve(zntt_ctx *c, size_t max_log_n) {
     st = dev_alloc(c, consts, (void **)&c->resv_consts);
     if (!st) c->resv_consts_bytes = consts;
   }
-  // The page-locked ring of large host-buffer copies.
-  if (!st && ring_wanted(n * ub)) st = ring_ensure(c);
+  // The page-locked ring of large host-buffer copies (optional: without it
+  // the calls copy from pageable memory, more slowly).
+  if (!st && ring_wanted(n * ub)) (void)ring_ensure(c);
   for (ResvSlot &sl : c->resv) {
     if (st || sl.bytes >= n * db) continue;
     uint32_t *t = nullptr;
