@@ -340,7 +340,7 @@ def run_fr_uneven(args, Z, ctx):
     n_total = sum(1 << L for L, _ in spec)
     muls = sum(((1 << L) // 2 * L + ((1 << L) if d else 0)) * MULS_PER_MODMUL for L, d in spec)
     b = Z.Batch(ctx, field, items)
-    tuned_ms = b.autotune(reps=2) if not args.no_tune else None
+    tuned_ms = b.autotune(reps=5) if not args.no_tune else None
     ms = timed(lambda: b.execute(stream=sp), args.steps, args.warmup, stream)
     b.status(stream=sp)
 
