@@ -168,6 +168,12 @@ int ensure_pinned(zntt_ctx *c, size_t bytes);
 bool ring_wanted(size_t bytes);
 int ring_ensure(zntt_ctx *c);
 void ring_free(zntt_ctx *c);
+// A ring of one's own (the serving loop's result reads): null on failure.
+// The copies hold no lock; the caller serialises them.
+HostRing *ring_create();
+void ring_destroy(HostRing *r);
+int ring_copy_to_device(HostRing *r, void *dev, const void *host, size_t bytes, hipStream_t stream);
+int ring_copy_to_host(HostRing *r, void *host, const void *dev, size_t bytes, hipStream_t stream);
 int copy_to_device(zntt_ctx *c, void *dev, const void *host, size_t bytes, hipStream_t stream);
 int copy_to_host(zntt_ctx *c, void *host, const void *dev, size_t bytes, hipStream_t stream);
 int power_table(zntt_ctx *c, int field, const Elem &root, int L, hipStream_t s, TwiddleEntry **out);

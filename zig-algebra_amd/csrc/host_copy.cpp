@@ -115,46 +115,48 @@ struct HostRing {
 
 bool ring_wanted(size_t bytes) { return bytes > kChunk; }
 
-int ring_ensure(zntt_ctx *c) {
-  if (c->ring) return ZNTT_OK;
-  if (c->ring_failed) return ZNTT_OUT_OF_MEMORY;
+HostRing *ring_create() {
   HostRing *r = nullptr;
   try {
     r = new HostRing();  // starts the copy threads
   } catch (...) {
-    c->ring_failed = true;  // no thread or memory for them: the plain copy serves
-    return ZNTT_OUT_OF_MEMORY;
+    return nullptr;  // no thread or memory for them
   }
   hipError_t e = hipHostMalloc((void **)&r->buf, (size_t)kSlots * kChunk, hipHostMallocDefault);
   for (int s = 0; s < kSlots && e == hipSuccess; s++) e = hipEventCreateWithFlags(&r->ev[s], hipEventDisableTiming);
   if (e != hipSuccess) {
     hipGetLastError();
-    for (auto &ev : r->ev)
-      if (ev) hipEventDestroy(ev);
-    if (r->buf) hipHostFree(r->buf);
-    delete r;
-    c->ring_failed = true;
-    return hip_status(e);
+    ring_destroy(r);
+    return nullptr;
   }
-  c->ring = r;
-  return ZNTT_OK;
+  return r;
+}
+
+// No copy may be using the ring.
+void ring_destroy(HostRing *r) {
+  if (!r) return;
+  for (auto &ev : r->ev)
+    if (ev) hipEventDestroy(ev);
+  if (r->buf) hipHostFree(r->buf);
+  delete r;
+}
+
+int ring_ensure(zntt_ctx *c) {
+  if (c->ring) return ZNTT_OK;
+  if (c->ring_failed) return ZNTT_OUT_OF_MEMORY;
+  c->ring = ring_create();
+  c->ring_failed = c->ring == nullptr;  // the plain copy serves until the next trim
+  return c->ring ? ZNTT_OK : ZNTT_OUT_OF_MEMORY;
 }
 
 // The caller has synchronised the context's stream (no copy uses the ring).
 void ring_free(zntt_ctx *c) {
   c->ring_failed = false;
-  HostRing *r = c->ring;
-  if (!r) return;
-  for (auto &ev : r->ev) hipEventDestroy(ev);
-  hipHostFree(r->buf);
-  delete r;
+  ring_destroy(c->ring);
   c->ring = nullptr;
 }
 
-int copy_to_device(zntt_ctx *c, void *dev, const void *host, size_t bytes, hipStream_t stream) {
-  if (!ring_wanted(bytes) || ring_ensure(c) != ZNTT_OK)
-    return hip_status(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, stream));
-  HostRing *r = c->ring;
+int ring_copy_to_device(HostRing *r, void *dev, const void *host, size_t bytes, hipStream_t stream) {
   const uint8_t *h = (const uint8_t *)host;
   uint8_t *d = (uint8_t *)dev;
   hipError_t e = hipSuccess;
@@ -170,13 +172,7 @@ int copy_to_device(zntt_ctx *c, void *dev, const void *host, size_t bytes, hipSt
   return hip_status(e);
 }
 
-int copy_to_host(zntt_ctx *c, void *host, const void *dev, size_t bytes, hipStream_t stream) {
-  if (!ring_wanted(bytes) || ring_ensure(c) != ZNTT_OK) {
-    hipError_t e = hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(stream);
-    return hip_status(e);
-  }
-  HostRing *r = c->ring;
+int ring_copy_to_host(HostRing *r, void *host, const void *dev, size_t bytes, hipStream_t stream) {
   uint8_t *h = (uint8_t *)host;
   const uint8_t *d = (const uint8_t *)dev;
   const size_t chunks = (bytes + kChunk - 1) / kChunk;
@@ -202,6 +198,21 @@ int copy_to_host(zntt_ctx *c, void *host, const void *dev, size_t bytes, hipStre
   }
   hipError_t e2 = hipStreamSynchronize(stream);
   return hip_status(e != hipSuccess ? e : e2);
+}
+
+int copy_to_device(zntt_ctx *c, void *dev, const void *host, size_t bytes, hipStream_t stream) {
+  if (!ring_wanted(bytes) || ring_ensure(c) != ZNTT_OK)
+    return hip_status(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, stream));
+  return ring_copy_to_device(c->ring, dev, host, bytes, stream);
+}
+
+int copy_to_host(zntt_ctx *c, void *host, const void *dev, size_t bytes, hipStream_t stream) {
+  if (!ring_wanted(bytes) || ring_ensure(c) != ZNTT_OK) {
+    hipError_t e = hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    return hip_status(e);
+  }
+  return ring_copy_to_host(c->ring, host, dev, bytes, stream);
 }
 
 }  // namespace zntt_detail

@@ -117,6 +117,13 @@ struct zntt_server {
   bool stop = false;
   std::thread thread;
   int tile_log = 11, kmax = 9;
+  // Result reads of more than one ring slot: a page-locked ring and a stream
+  // of their own, so a read neither takes the context's lock nor waits behind
+  // the scheduler's steps (made at the first such read).
+  std::mutex read_mu;
+  zntt_detail::HostRing *read_ring = nullptr;
+  hipStream_t read_stream = nullptr;
+  bool read_ring_failed = false;
 };
 
 namespace {
@@ -842,6 +849,15 @@ int zntt_server_read(zntt_server *s, uint64_t ticket, void *host, size_t len) {
   }
   // The request is done: its last step completed before it was marked so.
   hipSetDevice(s->ctx->device);
+  if (ring_wanted(bytes)) {
+    std::lock_guard<std::mutex> lk(s->read_mu);
+    if (!s->read_ring && !s->read_ring_failed) {
+      if (hipStreamCreateWithFlags(&s->read_stream, hipStreamNonBlocking) == hipSuccess)
+        s->read_ring = ring_create();
+      s->read_ring_failed = s->read_ring == nullptr;
+    }
+    if (s->read_ring) return ring_copy_to_host(s->read_ring, host, src, bytes, s->read_stream);
+  }
   return hip_status(hipMemcpy(host, src, bytes, hipMemcpyDeviceToHost));
 }
 
@@ -950,6 +966,8 @@ void zntt_server_destroy(zntt_server *s) {
     s->all.clear();
   }
   s->dev.reset();
+  ring_destroy(s->read_ring);
+  if (s->read_stream) hipStreamDestroy(s->read_stream);
   zntt_ctx *ctx = s->ctx;
   delete s;
   if (ctx) ctx_release(ctx);
