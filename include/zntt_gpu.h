@@ -141,6 +141,12 @@ size_t zntt_element_size(int field);
 /* ------------------------------------------------------------------------ */
 /* The seven zig-ntt functions (host buffers, synchronous, in place).        */
 /* ------------------------------------------------------------------------ */
+/* Host buffers over 16 MiB move through the context's 64 MiB of page-locked
+ * host memory, copied by three threads of the context while the DMA engine
+ * moves the previous part; both are made at the first such call (or by
+ * zntt_ctx_reserve at a size that needs them) and freed by zntt_ctx_trim
+ * (unless reserved) or zntt_ctx_destroy.  Without them the call copies from
+ * the caller's pageable memory directly, more slowly. */
 
 /* bitReverse(F, data) -- root.zig:38-58 */
 int zntt_bit_reverse(zntt_ctx *ctx, int field, void *data, size_t len);
@@ -407,7 +413,10 @@ int zntt_server_wait(zntt_server *s, uint64_t ticket, double timeout_ms);
 int zntt_server_cancel(zntt_server *s, uint64_t ticket);
 /* Copies a finished request's evaluations (kept or caller-owned) to host
  * memory: `len` elements, which must be 2^(log_n+log_blowup)
- * (ZNTT_LENGTH_MISMATCH otherwise).  Do not release the request meanwhile. */
+ * (ZNTT_LENGTH_MISMATCH otherwise).  Do not release the request meanwhile.
+ * Results over 16 MiB go through 64 MiB of page-locked host memory and a
+ * stream the server makes at its first such read; reads from several
+ * threads take turns there. */
 int zntt_server_read(zntt_server *s, uint64_t ticket, void *host, size_t len);
 /* Forgets a finished request and frees a kept result. */
 int zntt_server_release(zntt_server *s, uint64_t ticket);
