@@ -578,16 +578,17 @@ int upload_foreign(zntt_ctx *c, int field, size_t log_n, const void *const *stag
   if (st) return st;
   uint32_t *dev = (uint32_t *)c->aux;
   uint8_t *user = (uint8_t *)c->aux + n * db;
-  std::vector<uint8_t> host(n * ub);
-  for (size_t k = 0; k < log_n; k++) memcpy(host.data() + (((size_t)1 << k) - 1) * ub, stages[k], ((size_t)1 << k) * ub);
+  // Stage by stage from the caller's buffers into the flat user-layout
+  // staging (stage k at 2^k - 1), then the trailing one.
   const Elem one = f_one(field);
-  memcpy(host.data() + (n - 1) * ub, one.w, ub);
-  st = copy_to_device(c, user, host.data(), n * ub, s);
+  for (size_t k = 0; k < log_n && !st; k++)
+    st = copy_to_device(c, user + (((size_t)1 << k) - 1) * ub, stages[k], ((size_t)1 << k) * ub, s);
+  if (!st) st = copy_to_device(c, user + (n - 1) * ub, one.w, ub, s);
+  hipError_t e = st ? hipSuccess : zntt::launch_table_from_user(field, n, dev, (const uint32_t *)user, s);
+  const hipError_t e2 = hipStreamSynchronize(s);  // `one` goes out of scope
   if (st) return st;
-  hipError_t e = zntt::launch_table_from_user(field, n, dev, (const uint32_t *)user, s);
-  if (e == hipSuccess) e = hipStreamSynchronize(s);  // `host` goes out of scope
   *out = dev;
-  return hip_status(e);
+  return hip_status(e != hipSuccess ? e : e2);
 }
 
 // Checks a *_with_twiddles call's arguments in the reference's order
@@ -909,15 +910,17 @@ int zntt_precompute_twiddles(zntt_ctx *c, int field, size_t log_n, const void *r
                         src->table + (((size_t)1 << src->L) - 1) * zntt::dev_words(field), dev_bytes(field),
                         hipMemcpyDeviceToDevice, c->stream);
   if (he == hipSuccess) he = zntt::launch_table_to_user(field, n - 1, (uint32_t *)c->staging, e.table, c->stream);
-  std::vector<uint8_t> host((n - 1) * ub);
+  // Stage s of the flat user-layout table (at 2^s - 1) straight into the
+  // caller's stage buffer.
   int cst = hip_status(he);
-  if (!cst) cst = copy_to_host(c, host.data(), c->staging, (n - 1) * ub, c->stream);
+  for (size_t s = 0; s < log_n && !cst; s++)
+    cst = copy_to_host(c, stages[s], (uint8_t *)c->staging + (((size_t)1 << s) - 1) * ub, ((size_t)1 << s) * ub,
+                       c->stream);
   if (cst) {
     hipStreamSynchronize(c->stream);
     dev_free(c, e.table, e.bytes);
     return cst;
   }
-  for (size_t s = 0; s < log_n; s++) memcpy(stages[s], host.data() + (((size_t)1 << s) - 1) * ub, ((size_t)1 << s) * ub);
   e.key.assign(stages, stages + log_n);
   e.fingerprint = table_fingerprint(field, log_n, (const void *const *)stages);
   e.reg_id = c->next_reg_id++;
