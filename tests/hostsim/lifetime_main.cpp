@@ -410,6 +410,14 @@ void host_copies_through_the_ring() {
   CHECK(zntt_lde(c, F, coeffs.data(), log_k, log_b, sh.b, out.data()));
   EXPECT(memcmp(out.data(), coeffs.data(), coeffs.size()) == 0);
   for (size_t i = coeffs.size(); i < out.size(); i++) EXPECT(out[i] == 0);
+  // Lengths that end in a partial slot, both ways (a polynomial product:
+  // 600,001 and 700,003 coefficients in, 1,300,003 out).
+  const size_t na = 600001, nb = 700003;
+  std::vector<uint8_t> a(na * 32, 1), b(nb * 32, 2), prod((na + nb - 1) * 32);
+  a[na * 32 - 1] = b[nb * 32 - 1] = 3;  // nonzero leading coefficients
+  size_t got = 0;
+  CHECK(zntt_poly_mul(c, F, a.data(), na, b.data(), nb, prod.data(), na + nb - 1, &got));
+  EXPECT(got == na + nb - 1);
   zntt_ctx_destroy(c);
 }
 

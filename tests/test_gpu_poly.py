@@ -85,3 +85,24 @@ def test_large_poly_mul_by_evaluation(Z, ctx, field):
     torch.cuda.synchronize()
     got = tout.cpu().numpy().view(np.uint64).reshape(r.shape)
     assert np.array_equal(got, r)
+
+
+@pytest.mark.parametrize("field", [O.FIELD_FR, O.FIELD_GL], ids=["fr", "gl"])
+def test_poly_mul_of_lengths_off_the_copy_slots(Z, ctx, field):
+    """Host buffers of lengths that end in a partial 16 MiB slot of the
+    page-locked copy ring, both ways (BN254 Fr: 600,001 and 700,003
+    coefficients in, 1,300,003 out; Goldilocks four times as many):
+    (a*b)(z) = a(z) b(z) at three points."""
+    scale = 1 if field == O.FIELD_FR else 4
+    na, nb = 600001 * scale, 700003 * scale
+    a = O.gen_uniform(field, 71, na)
+    b = O.gen_uniform(field, 72, nb)
+    r = ctx.poly_mul(field, a, b)
+    assert r.shape[0] == na + nb - 1
+    p = O.MODULUS[field]
+    for z in (5, 987654321, p - 3):
+        zz = O.from_int(field, z)
+        lhs = O.value(field, O.poly_eval(field, r, zz))
+        rhs = O.value(field, O.poly_eval(field, a, zz)) * O.value(field, O.poly_eval(field, b, zz)) % p
+        assert lhs == rhs, z
+
